@@ -1,0 +1,210 @@
+"""Throughput of the explicit-dynamics step on one MI355X.
+
+Workload: the 1.57M-tet cube -- 64^3 hexes x 6 tet4 (1,572,864 elements,
+274,625 nodes), interior nodes jittered by U(-0.1, 0.1) of the spacing from
+SplitMix64 seed 0, aluminium J2 elastoplastic (E 68.9 GPa, nu 0.33,
+rho 2700, Y0 276 MPa, H 700 MPa), impact at -10 m/s on a rigid wall at z = 0
+with penalty k = 10 E h, dt = 0.5 x the stable step.  A step is one loop
+iteration of Dynamics::solve over the whole mesh, inputs resident in HBM.
+
+Two runs of the same workload: the plain FP64 step (exact radial return) and
+the mixed-precision step (FP32 element kernels, FP64 nodal state, mass, dt
+and energies; MLP surrogate with exact fallback when --surrogate is given and
+weights are installed).  The headline value is the mixed-precision run; the
+plain run, the speedup and the displacement error between them ride along.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+N_HEX = 64
+SIDE = 0.25
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--scatter", default="block", choices=["block", "color", "atomic"])
+    ap.add_argument("--surrogate", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--n", type=int, default=N_HEX, help="hexes per side (default 64)")
+    return ap.parse_args()
+
+
+def make_mesh(n):
+    import cppimpact_amd as ci
+    return ci.box_mesh(n, n, n, SIDE, SIDE, SIDE, origin=(0.0, 0.0, 1e-6), jitter=0.1, seed=0)
+
+
+def run_mode(x, conn, args, precision, stress_update, dt):
+    import numpy as np
+
+    import cppimpact_amd as ci
+    from cppimpact_amd import _lib as L
+
+    E = 68.9e9
+    mesh = ci.Mesh.from_arrays(x, conn)
+    mat = ci.Elastoplastic(E, 2700.0, 0.33, 1.0, 700e6, 276e6)
+    wall = ci.Wall("Wall", 0.0, 10 * E * SIDE / args.n, None, 0, 1)
+    scatter = {"block": L.SCATTER_BLOCK, "color": L.SCATTER_COLOR, "atomic": L.SCATTER_ATOMIC}
+    dyn = ci.Dynamics(mesh, mat, wall, physics=L.PHYSICS_EP, precision=precision,
+                      stress_update=stress_update, scatter=scatter[args.scatter])
+    dyn.vel[:] = (0.0, 0.0, -10.0)
+    t0 = time.time()
+    dyn.init(dt)
+    setup = time.time() - t0
+    dyn.step(args.warmup)
+    dyn.synchronize()
+    t0 = time.perf_counter()
+    gpu_ms = dyn.time_steps(args.steps)
+    dyn.synchronize()
+    wall_s = time.perf_counter() - t0
+    prof = dyn.profile_kernels(args.profile_steps)
+    dyn.synchronize()
+    info = dyn.info()
+    xg, *_ = dyn.fetch()
+    energy, step = dyn.energy()
+    _, ep = dyn.element_state()
+    dyn.close()
+    ne = info.num_elements
+    elem_name = [k for k in prof if k.startswith("elem")][0]
+    node_name = [k for k in prof if k.startswith("node")][0]
+    elem_ms = prof[elem_name][0] / prof[elem_name][1]
+    node_ms = prof[node_name][0] / prof[node_name][1]
+    return dict(
+        dt=info.dt, setup_s=setup, wall_s=wall_s, gpu_ms=gpu_ms,
+        value=ne * args.steps / wall_s, ms_per_step=1e3 * wall_s / args.steps,
+        gpu_ms_per_step=gpu_ms / args.steps,
+        elem_kernel=elem_name, elem_ms=elem_ms, node_ms=node_ms,
+        elem_bytes=info.elem_kernel_bytes, node_bytes=info.node_kernel_bytes,
+        step_bytes=info.bytes_per_step, x=xg, energy=energy.tolist(), step=step,
+        epbar_max=float(np.max(ep)), blocks=info.num_blocks,
+        boundary_nodes=info.num_boundary_nodes, colors=info.num_colors,
+    )
+
+
+def roofline(r):
+    achieved = r["elem_bytes"] / (r["elem_ms"] * 1e-3) / 1e9
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "elem_block_cube64_traffic.json")
+    if os.path.exists(path):
+        try:
+            traffic = json.load(open(path)).get(r["elem_kernel_key"])
+        except Exception:
+            traffic = None
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": r["elem_kernel"], "kernel_ms": round(r["elem_ms"], 5),
+            "algorithmic_bytes": r["elem_bytes"]}
+
+
+def cpu_baseline(x, conn, dt, nsteps):
+    import numpy as np
+
+    from oracle import pyoracle as O
+    E = 68.9e9
+    wall = dict(location=0.0, stiffness=10 * E * SIDE / N_HEX)
+    s = O.Solver(x, conn, (0.0, 0.0, -10.0), E=E, nu=0.33, rho=2700.0, physics=O.COMPAT,
+                 wall=wall, dt=dt)
+    out = s.solve(nsteps, with_energy=False)
+    secs = out["loop_seconds"]
+    return {"value": round(conn.shape[0] * nsteps / secs, 1), "unit": "element-steps/s",
+            "cores": 1, "kind": "port",
+            "sample": f"{nsteps} loop iteration(s) of the reference algorithm (compat: per-element "
+                      f"K_e = sum_q w detJ B^T D B on the 5-point rule, f = K_e du) over the full "
+                      f"{conn.shape[0]}-element mesh, 1 thread, initialisation excluded "
+                      f"({secs:.1f} s)",
+            "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} / WORLD_SIZE {world} "
+              "is not supported (the step would shard by element partition with an "
+              "interface-force exchange, not built here)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    from cppimpact_amd import _lib as L
+
+    x, conn = make_mesh(args.n)
+    print(f"bench: mesh {conn.shape[0]} tet4, {x.shape[0]} nodes", file=sys.stderr, flush=True)
+    # one dt for both runs: half the stable step of the FP64 context
+    plain = run_mode(x, conn, args, L.PREC_F64, L.RETURN_EXACT, 0.0)
+    plain["elem_kernel_key"] = "f64"
+    print(f"bench: fp64 {plain['value']:.4g} el-steps/s, elem kernel {plain['elem_ms']:.4f} ms",
+          file=sys.stderr, flush=True)
+    ret = L.RETURN_MLP if args.surrogate else L.RETURN_EXACT
+    mixed = run_mode(x, conn, args, L.PREC_F32, ret, plain["dt"])
+    mixed["elem_kernel_key"] = "f32_mlp" if args.surrogate else "f32"
+    print(f"bench: fp32 {mixed['value']:.4g} el-steps/s, elem kernel {mixed['elem_ms']:.4f} ms",
+          file=sys.stderr, flush=True)
+    disp = np.abs(plain["x"] - x).max()
+    derr = float(np.abs(mixed["x"] - plain["x"]).max() / disp) if disp > 0 else 0.0
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(x, conn, plain["dt"], args.cpu_steps)
+        print(f"bench: cpu baseline {cpu['value']:.4g} el-steps/s", file=sys.stderr, flush=True)
+    line = {
+        "metric": "element-steps/s, 1 MI355X: FP32+surrogate vs FP64 plain at <=1e-3 disp err",
+        "value": round(mixed["value"], 1),
+        "unit": "element-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cube {args.n}^3 hexes x 6 tet4 ({conn.shape[0]} elements, "
+                        f"{x.shape[0]} nodes), jitter 0.1 seed 0, J2 ep, v0 -10 m/s, "
+                        f"wall k=10*E*h, dt=0.5*stable",
+            "mode": "fp32 element kernels" + (" + MLP surrogate" if args.surrogate else "")
+                    + ", fp64 nodal state/mass/dt/energy",
+            "scatter": args.scatter,
+            "dt": plain["dt"],
+        },
+        "roofline": roofline(mixed),
+        "cpu_baseline": cpu,
+        "plain_fp64": {"value": round(plain["value"], 1),
+                       "ms_per_step": round(plain["ms_per_step"], 5),
+                       "roofline": roofline(plain)},
+        "speedup_vs_fp64": round(mixed["value"] / plain["value"], 4),
+        "disp_rel_err_vs_fp64": derr,
+        "gpu_event_ms_per_step": round(mixed["gpu_ms_per_step"], 5),
+        "node_kernel_ms": round(mixed["node_ms"], 5),
+        "blocks": mixed["blocks"], "boundary_nodes": mixed["boundary_nodes"],
+        "epbar_max": mixed["epbar_max"],
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

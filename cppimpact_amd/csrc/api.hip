@@ -1,0 +1,921 @@
+// Context management and the C ABI (include/cppimpact_hip.h) of
+// libcppimpact_hip.so: device buffers, uploads/downloads between the
+// reference's host layouts and the device layout, the hipGraph-captured
+// step, timing and diagnostics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cppimpact_hip.h"
+#include "device.h"
+#include "internal.h"
+#include "kernels.h"
+#include "mlp_host.h"
+
+namespace cpi {
+void set_global_error(const std::string &s);
+const char *global_error();
+}  // namespace cpi
+
+using namespace cpi;
+
+struct cpi_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nn = 0, ne = 0, npe = 0, nq = 0, rule = 0, phys = 0, prec = 0, scatter = 0, mass_kind = 0;
+  cpi_options opt{};
+  cpi_material mat{};
+  cpi_wall wall{};
+  std::vector<double> X0u, V0u;  // user order [N][3]
+  std::vector<int32_t> conn_u, color, slave_u;
+  int ncol = 0;
+  std::vector<int32_t> eperm, nperm, ninv;  // device->user elem, device->user node, user->device
+  std::vector<int32_t> color_off;           // colour scatter launch ranges (device order)
+  BlockPlan plan;
+  std::vector<void *> allocs;
+  double *d_X0[3] = {nullptr, nullptr, nullptr};
+  double *d_tmp = nullptr;
+  size_t tmp_len = 0;
+  Params P{};
+  int nblocks = 0, n_elem_rows = 0, n_node_rows = 0, n_rows = 0;
+  std::vector<int> color_rows;  // first energy row of each colour launch
+  size_t smem = 0;
+  hipGraphExec_t g_plain = nullptr, g_aux = nullptr;
+  bool mass_ready = false, inited = false;
+  double dt = 0.0;
+  int64_t step = 0;
+  MlpHost mlp;  // surrogate weights and sample buffer
+};
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                                  \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      (ctx)->err = std::string(#expr) + " failed at " __FILE__ ":" + std::to_string(__LINE__) + \
+                   ": " + hipGetErrorString(e_);                                          \
+      return CPI_ERR_HIP;                                                                 \
+    }                                                                                     \
+  } while (0)
+
+int fail(cpi_ctx *c, int code, const std::string &msg) {
+  c->err = msg;
+  return code;
+}
+
+template <typename T>
+int dalloc(cpi_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess) {
+    c->err = "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " +
+             hipGetErrorString(e);
+    return CPI_ERR_ALLOC;
+  }
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return CPI_OK;
+}
+
+template <typename T>
+int upload(cpi_ctx *c, T *d, const std::vector<T> &h) {
+  if (h.empty()) return CPI_OK;
+  HIPCHK(c, hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return CPI_OK;
+}
+
+int default_rule(int npe, int phys, int q) {
+  if (q >= 0) return q;
+  if (phys == CPI_PHYSICS_COMPAT) return CPI_QUAD_REF5;
+  return npe == 4 ? CPI_QUAD_CENTROID : CPI_QUAD_EXACT4;
+}
+
+double rule_weight_sum(int rule) {
+  switch (rule) {
+    case CPI_QUAD_REF5: return -0.13333333333 + 0.075 + 0.075 + 0.075 + 0.075;
+    case CPI_QUAD_REF4: return 0.041666667 + 0.041666667 + 0.041666667 + 0.041666667;
+    case CPI_QUAD_EXACT4: return 4.0 / 24.0;
+    default: return 1.0 / 6.0;
+  }
+}
+
+// user AoS [N][3] -> device SoA (device node order)
+int put_nodes(cpi_ctx *c, double *const d[3], const double *aos) {
+  std::vector<double> tmp(c->nn);
+  for (int k = 0; k < 3; k++) {
+    for (int i = 0; i < c->nn; i++) tmp[i] = aos[3L * c->nperm[i] + k];
+    HIPCHK(c, hipMemcpyAsync(d[k], tmp.data(), sizeof(double) * c->nn, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return CPI_OK;
+}
+
+int get_nodes(cpi_ctx *c, double *const d[3], double *aos) {
+  std::vector<double> tmp(c->nn);
+  for (int k = 0; k < 3; k++) {
+    HIPCHK(c, hipMemcpyAsync(tmp.data(), d[k], sizeof(double) * c->nn, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < c->nn; i++) aos[3L * c->nperm[i] + k] = tmp[i];
+  }
+  return CPI_OK;
+}
+
+int enqueue_step(cpi_ctx *c, bool aux) {
+  Params p = c->P;
+  p.store_aux = aux ? 1 : 0;
+  hipError_t e = hipSuccess;
+  if (c->scatter == CPI_SCATTER_BLOCK) {
+    p.epart_base = 0;
+    e = launch_elem_block(p, c->rule, c->prec, c->phys, aux, c->nblocks, c->smem, c->stream);
+    if (e == hipSuccess) {
+      p.epart_base = c->n_elem_rows;
+      e = launch_node_bnd(p, aux, c->stream);
+    }
+  } else {
+    const bool atomic = c->scatter == CPI_SCATTER_ATOMIC;
+    const int nc = atomic ? 1 : c->ncol;
+    for (int k = 0; k < nc && e == hipSuccess; k++) {
+      p.elem_base = atomic ? 0 : c->color_off[k];
+      p.elem_count = atomic ? c->ne : c->color_off[k + 1] - c->color_off[k];
+      p.epart_base = c->color_rows[k];
+      e = launch_elem_direct(p, c->rule, c->prec, c->phys, atomic, c->stream);
+    }
+    if (e == hipSuccess) {
+      p.epart_base = c->n_elem_rows;
+      e = launch_node_all(p, aux, c->stream);
+    }
+  }
+  if (e == hipSuccess && aux) e = launch_energy_reduce(p, c->stream);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return CPI_OK;
+}
+
+int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out) {
+  hipGraph_t g = nullptr;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(c, aux);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  return CPI_OK;
+}
+
+void drop_graphs(cpi_ctx *c) {
+  if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
+  if (c->g_aux) (void)hipGraphExecDestroy(c->g_aux);
+  c->g_plain = c->g_aux = nullptr;
+}
+
+int ensure_tmp(cpi_ctx *c, size_t n) {
+  if (c->tmp_len >= n) return CPI_OK;
+  double *p = nullptr;
+  int rc = dalloc(c, &p, n);
+  if (rc) return rc;
+  c->d_tmp = p;
+  c->tmp_len = n;
+  return CPI_OK;
+}
+
+int compute_mass(cpi_ctx *c) {
+  Params p = c->P;
+  HIPCHK(c, hipMemsetAsync((void *)p.m, 0, sizeof(double) * c->nn, c->stream));
+  const double rho = c->mat.rho;
+  hipError_t e = hipSuccess;
+  if (c->scatter == CPI_SCATTER_BLOCK) {
+    e = launch_mass(p, c->rule, 0, c->mass_kind, rho, (double *)p.m, c->nblocks, c->stream);
+  } else if (c->scatter == CPI_SCATTER_ATOMIC) {
+    p.elem_base = 0;
+    p.elem_count = c->ne;
+    e = launch_mass(p, c->rule, 2, c->mass_kind, rho, (double *)p.m, 0, c->stream);
+  } else {
+    for (int k = 0; k < c->ncol && e == hipSuccess; k++) {
+      p.elem_base = c->color_off[k];
+      p.elem_count = c->color_off[k + 1] - c->color_off[k];
+      e = launch_mass(p, c->rule, 1, c->mass_kind, rho, (double *)p.m, 0, c->stream);
+    }
+  }
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("mass kernel: ") + hipGetErrorString(e));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mass_ready = true;
+  return CPI_OK;
+}
+
+int reset_state(cpi_ctx *c) {
+  int rc = put_nodes(c, c->P.x, c->X0u.data());
+  if (!rc) rc = put_nodes(c, c->P.v, c->V0u.data());
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) {
+    HIPCHK(c, hipMemsetAsync(c->P.f[k], 0, sizeof(double) * c->nn, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->P.acc[k], 0, sizeof(double) * c->nn, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->P.du[k], 0, sizeof(double) * c->nn, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->P.xprev[k], c->P.x[k], sizeof(double) * c->nn,
+                             hipMemcpyDeviceToDevice, c->stream));
+  }
+  const size_t rs = c->prec == CPI_PREC_F32 ? 4 : 8;
+  if (c->phys != CPI_PHYSICS_COMPAT) {
+    HIPCHK(c, hipMemsetAsync(c->P.sig, 0, rs * 6 * c->nq * (size_t)c->ne, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->P.epbar, 0, rs * c->nq * (size_t)c->ne, c->stream));
+  }
+  HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return CPI_OK;
+}
+
+// stable dt of the current device positions: element limit and the wall
+// spring limit 2 sqrt(m_min / k)
+int stable_dt(cpi_ctx *c, double *dte, double *dtc) {
+  const int nb = (c->ne + 255) / 256;
+  int rc = ensure_tmp(c, std::max(nb, c->nn));
+  if (rc) return rc;
+  hipError_t e = launch_elem_dt(c->P, c->d_tmp, c->stream);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
+  std::vector<double> h(std::max(nb, c->nn));
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tmp, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double m1 = 1e300;
+  for (int i = 0; i < nb; i++) m1 = std::min(m1, h[i]);
+  *dte = m1;
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->P.m, sizeof(double) * c->nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double mmin = 1e300;
+  for (int i = 0; i < c->nn; i++)
+    if (h[i] > 0.0) mmin = std::min(mmin, h[i]);
+  *dtc = (c->wall.enabled && c->wall.stiffness > 0.0) ? 2.0 * std::sqrt(mmin / c->wall.stiffness)
+                                                      : INFINITY;
+  return CPI_OK;
+}
+
+int check_flags(cpi_ctx *c) {
+  Ctl h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->P.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h.flags & 2u) return fail(c, CPI_ERR_NUMERIC, "inverted or degenerate element (detJ <= 0) during the step");
+  if (h.flags & 1u) return fail(c, CPI_ERR_NUMERIC, "non-finite value in the energy terms (NaN/Inf state)");
+  return CPI_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" void cpi_default_options(cpi_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->physics = CPI_PHYSICS_COMPAT;
+  o->precision = CPI_PREC_F64;
+  o->scatter = CPI_SCATTER_BLOCK;
+  o->quadrature = CPI_QUAD_DEFAULT;
+  o->mass = CPI_MASS_DEFAULT;
+  o->stress_update = CPI_RETURN_EXACT;
+  o->use_graph = 1;
+  o->track_energy = 1;
+  o->contact_slave_only = 0;
+  o->block_elements = 0;
+  o->device = 0;
+  o->gravity[0] = 0.0;
+  o->gravity[1] = 0.0;
+  o->gravity[2] = -9.81;
+  o->mlp_tolerance = 1e-3;
+}
+
+extern "C" const char *cpi_last_error(const cpi_ctx *c) {
+  return c ? c->err.c_str() : cpi::global_error();
+}
+
+extern "C" void cpi_destroy(cpi_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  for (void *p : c->allocs) (void)hipFree(p);
+  mlp_free(c->mlp);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *mat,
+                       const cpi_wall *wall, const cpi_options *opt) {
+  c->opt = *opt;
+  c->mat = *mat;
+  if (wall) c->wall = *wall;
+  c->nn = m->num_nodes;
+  c->ne = m->num_elements;
+  c->npe = m->nodes_per_element;
+  const int npe = c->npe;
+  if (c->nn <= 0 || c->ne <= 0 || (npe != 4 && npe != 10) || !m->xloc || !m->element_nodes)
+    return fail(c, CPI_ERR_INVALID, "cpi_create: empty mesh or nodes_per_element not 4/10");
+  if (opt->physics < 0 || opt->physics > 1 || opt->precision < 0 || opt->precision > 1 ||
+      opt->scatter < 0 || opt->scatter > 2 || opt->stress_update < 0 || opt->stress_update > 1)
+    return fail(c, CPI_ERR_INVALID, "cpi_create: option out of range");
+  if (!(mat->E > 0) || !(mat->rho > 0) || !(mat->nu > -1.0 && mat->nu < 0.5))
+    return fail(c, CPI_ERR_INVALID, "cpi_create: material needs E > 0, rho > 0, -1 < nu < 0.5");
+  if (wall && wall->enabled && (wall->dim < 0 || wall->dim > 2 || (wall->normal != 1 && wall->normal != -1)))
+    return fail(c, CPI_ERR_INVALID, "cpi_create: wall dim must be 0..2 and normal +-1");
+  c->phys = opt->physics;
+  if (c->phys == CPI_PHYSICS_EP && opt->stress_update == CPI_RETURN_MLP) {
+    if (npe != 4 || opt->precision != CPI_PREC_F32 || opt->scatter != CPI_SCATTER_BLOCK)
+      return fail(c, CPI_ERR_INVALID, "the MLP surrogate runs on tet4, FP32, block scatter");
+    c->phys = 2;
+  }
+  c->prec = opt->precision;
+  c->scatter = opt->scatter;
+  c->rule = default_rule(npe, opt->physics, opt->quadrature);
+  if (c->rule < 0 || c->rule > 3) return fail(c, CPI_ERR_INVALID, "cpi_create: bad quadrature");
+  if (npe == 10 && c->rule == CPI_QUAD_CENTROID)
+    return fail(c, CPI_ERR_INVALID, "tet10 needs a 4- or 5-point rule");
+  c->mass_kind = opt->mass >= 0 ? opt->mass
+                                 : (opt->physics == CPI_PHYSICS_COMPAT ? CPI_MASS_ROWSUM : CPI_MASS_HRZ);
+  c->nq = (npe == 4) ? 1 : (c->rule == CPI_QUAD_REF5 ? 5 : 4);
+  // host copies and validation
+  c->X0u.assign(m->xloc, m->xloc + 3L * c->nn);
+  c->V0u.assign(3L * c->nn, 0.0);
+  c->conn_u.assign(m->element_nodes, m->element_nodes + (long)npe * c->ne);
+  for (int32_t v : c->conn_u)
+    if (v < 0 || v >= c->nn) return fail(c, CPI_ERR_MESH, "element_nodes references a node out of range");
+  for (double v : c->X0u)
+    if (!std::isfinite(v)) return fail(c, CPI_ERR_MESH, "non-finite node coordinate");
+  for (int e = 0; e < c->ne; e++) {
+    const int32_t *ce = &c->conn_u[(long)npe * e];
+    const int o = npe == 4 ? 0 : 2, i1 = npe == 4 ? 1 : 0, i2 = npe == 4 ? 2 : 1;
+    double a[3], b[3], d[3];
+    for (int k = 0; k < 3; k++) {
+      a[k] = c->X0u[3L * ce[i1] + k] - c->X0u[3L * ce[o] + k];
+      b[k] = c->X0u[3L * ce[i2] + k] - c->X0u[3L * ce[o] + k];
+      d[k] = c->X0u[3L * ce[3] + k] - c->X0u[3L * ce[o] + k];
+    }
+    double v6 = a[0] * (b[1] * d[2] - b[2] * d[1]) - a[1] * (b[0] * d[2] - b[2] * d[0]) +
+                a[2] * (b[0] * d[1] - b[1] * d[0]);
+    if (!(v6 > 0.0))
+      return fail(c, CPI_ERR_MESH, "element " + std::to_string(e) + " is inverted or degenerate");
+  }
+  // colouring (always: reported by cpi_get_coloring)
+  c->color.assign(c->ne, 0);
+  c->ncol = greedy_coloring(c->ne, npe, c->conn_u.data(), c->nn, c->color.data());
+  // device
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt->device || opt->device < 0)
+    return fail(c, CPI_ERR_NO_DEVICE, "no HIP device " + std::to_string(opt->device));
+  HIPCHK(c, hipSetDevice(opt->device));
+  hipDeviceProp_t prop;
+  HIPCHK(c, hipGetDeviceProperties(&prop, opt->device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(c, CPI_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                          ", this library is built for gfx950 only");
+  c->dev = opt->device;
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // device orders
+  if (c->scatter == CPI_SCATTER_BLOCK) {
+    int me = opt->block_elements > 0 ? opt->block_elements : (npe == 4 ? 512 : 128);
+    // LDS budget: two workgroups per 160 KiB CU, never above the device's
+    // per-workgroup limit; node records take 64 bytes each
+    const size_t rsz = c->prec == CPI_PREC_F32 ? 4 : 8;
+    const long budget = std::min<long>((long)prop.sharedMemPerBlock, 81920);
+    const long forces = (long)rsz * 3 * npe * me;
+    const long red = 8L * EP_N * (elem_block_threads(npe) / 64);
+    long mnl = (budget - forces - red - 64) / 64;
+    if (mnl < 4L * npe)
+      return fail(c, CPI_ERR_INVALID, "block_elements " + std::to_string(me) +
+                                          " does not fit the LDS budget");
+    int mn = (int)std::min<long>(mnl, 1024);
+    std::string e;
+    if (!build_block_plan(c->ne, npe, c->conn_u.data(), c->nn, c->X0u.data(), me, mn, c->plan, e))
+      return fail(c, CPI_ERR_INVALID, "partition: " + e);
+    c->eperm = c->plan.elem_perm;
+    c->nperm = c->plan.node_perm;
+    c->ninv = c->plan.node_inv;
+    c->nblocks = c->plan.nblocks;
+  } else {
+    c->nperm.resize(c->nn);
+    c->ninv.resize(c->nn);
+    for (int i = 0; i < c->nn; i++) c->nperm[i] = c->ninv[i] = i;
+    if (c->scatter == CPI_SCATTER_COLOR) {
+      color_order(c->ne, c->color.data(), c->ncol, c->eperm, c->color_off);
+    } else {
+      c->eperm.resize(c->ne);
+      for (int e = 0; e < c->ne; e++) c->eperm[e] = e;
+    }
+  }
+  Params &P = c->P;
+  P.nn = c->nn;
+  P.ne = c->ne;
+  P.npe = npe;
+  P.nq = c->nq;
+  const double E = mat->E, nu = mat->nu;
+  const double k = E / ((1 + nu) * (1 - 2 * nu));
+  P.c11 = (1 - nu) * k;
+  P.c12 = nu * k;
+  P.c44 = ((1 - 2 * nu) / 2) * k;
+  P.mu = E / (2.0 * (1.0 + nu));
+  P.sy0 = mat->Y0;
+  P.H = mat->H;
+  P.wave_speed = std::sqrt(P.c11 / mat->rho);
+  P.wall_on = (wall && wall->enabled) ? 1 : 0;
+  P.wall_dim = wall ? wall->dim : 2;
+  P.wall_normal = wall ? wall->normal : 1;
+  P.wall_loc = wall ? wall->location : 0.0;
+  P.wall_k = wall ? wall->stiffness : 0.0;
+  P.wall_nk = wall ? wall->normal * wall->stiffness : 0.0;
+  for (int i = 0; i < 3; i++) P.grav[i] = opt->gravity[i];
+  P.wsum = rule_weight_sum(c->rule);
+  int rc = CPI_OK;
+  const int nn = c->nn, ne = c->ne;
+  for (int kk = 0; kk < 3 && !rc; kk++) {
+    rc = dalloc(c, &P.x[kk], nn);
+    if (!rc) rc = dalloc(c, &P.v[kk], nn);
+    if (!rc) rc = dalloc(c, &P.f[kk], nn);
+    if (!rc) rc = dalloc(c, &P.acc[kk], nn);
+    if (!rc) rc = dalloc(c, &P.du[kk], nn);
+    if (!rc) rc = dalloc(c, &P.xprev[kk], nn);
+    if (!rc) rc = dalloc(c, &c->d_X0[kk], nn);
+  }
+  double *dm = nullptr;
+  if (!rc) rc = dalloc(c, &dm, nn);
+  P.m = dm;
+  if (rc) return rc;
+  // connectivity [npe][ne] in device element and node order
+  {
+    std::vector<int32_t> cd((size_t)npe * ne);
+    for (int e = 0; e < ne; e++)
+      for (int j = 0; j < npe; j++)
+        cd[(size_t)j * ne + e] = c->ninv[c->conn_u[(size_t)npe * c->eperm[e] + j]];
+    int32_t *d = nullptr;
+    if ((rc = dalloc(c, &d, cd.size())) || (rc = upload(c, d, cd))) return rc;
+    P.conn = d;
+  }
+  if (opt->contact_slave_only) {
+    std::vector<uint8_t> mask(nn, 0);
+    for (int i = 0; i < m->num_slave_nodes; i++) {
+      int s = m->slave_nodes ? m->slave_nodes[i] : -1;
+      if (s < 0 || s >= nn) return fail(c, CPI_ERR_MESH, "slave node out of range");
+      mask[c->ninv[s]] = 1;
+    }
+    uint8_t *d = nullptr;
+    if ((rc = dalloc(c, &d, mask.size())) || (rc = upload(c, d, mask))) return rc;
+    P.contact_mask = d;
+  }
+  if (c->phys != CPI_PHYSICS_COMPAT) {
+    const size_t rs = c->prec == CPI_PREC_F32 ? 4 : 8;
+    char *s = nullptr, *q = nullptr;
+    if ((rc = dalloc(c, &s, rs * 6 * c->nq * (size_t)ne)) || (rc = dalloc(c, &q, rs * c->nq * (size_t)ne)))
+      return rc;
+    P.sig = s;
+    P.epbar = q;
+  }
+  if (c->scatter == CPI_SCATTER_BLOCK) {
+    BlockPlan &B = c->plan;
+    int32_t *meta, *bn, *bs, *sp;
+    uint16_t *cl, *cp, *ci;
+    double *part;
+    if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
+        (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||
+        (rc = dalloc(c, &cp, B.csr_ptr.size())) || (rc = upload(c, cp, B.csr_ptr)) ||
+        (rc = dalloc(c, &ci, B.csr_idx.size())) || (rc = upload(c, ci, B.csr_idx)) ||
+        (rc = dalloc(c, &bn, B.bnd_nodes.size())) || (rc = upload(c, bn, B.bnd_nodes)) ||
+        (rc = dalloc(c, &bs, B.bnd_slot.size())) || (rc = upload(c, bs, B.bnd_slot)) ||
+        (rc = dalloc(c, &sp, B.slot_ptr.size())) || (rc = upload(c, sp, B.slot_ptr)) ||
+        (rc = dalloc(c, &part, 3 * (size_t)std::max(B.n_slots, 1))))
+      return rc;
+    P.meta = meta;
+    P.conn_local = cl;
+    P.csr_ptr = cp;
+    P.csr_idx = ci;
+    P.bnd_nodes = bn;
+    P.bnd_slot = bs;
+    P.slot_ptr = sp;
+    P.partial = part;
+    P.n_interior = B.n_interior;
+    P.n_boundary = B.n_boundary;
+    P.max_loc = B.max_local_nodes;
+    P.max_el = B.max_block_elems;
+    c->smem = elem_block_smem(npe, c->prec, P.max_loc, P.max_el);
+    if (c->smem > (size_t)prop.sharedMemPerBlock)
+      return fail(c, CPI_ERR_INVALID, "element block needs " + std::to_string(c->smem) +
+                                          " bytes of LDS; lower block_elements");
+    hipError_t pe = prepare_elem_block(npe, c->rule, c->prec, c->phys, c->smem);
+    if (pe != hipSuccess)
+      return fail(c, CPI_ERR_HIP, std::string("LDS opt-in for the element kernel: ") +
+                                      hipGetErrorString(pe));
+    c->n_elem_rows = c->nblocks;
+    c->n_node_rows = std::max(1, (B.n_boundary + 255) / 256);
+  } else {
+    const int nt = npe == 4 ? 256 : 64;
+    c->color_rows.clear();
+    int rows = 0;
+    if (c->scatter == CPI_SCATTER_COLOR) {
+      for (int kk = 0; kk < c->ncol; kk++) {
+        c->color_rows.push_back(rows);
+        rows += (c->color_off[kk + 1] - c->color_off[kk] + nt - 1) / nt;
+      }
+    } else {
+      c->color_rows.push_back(0);
+      rows = (ne + nt - 1) / nt;
+    }
+    c->n_elem_rows = rows;
+    c->n_node_rows = (nn + 255) / 256;
+  }
+  c->n_rows = std::max(c->n_elem_rows + c->n_node_rows, (nn + 255) / 256);
+  double *ep = nullptr, *hist = nullptr;
+  Ctl *ctl = nullptr;
+  if ((rc = dalloc(c, &ep, (size_t)EP_N * c->n_rows)) || (rc = dalloc(c, &ctl, 1)) ||
+      (rc = dalloc(c, &hist, 8 * 4096)))
+    return rc;
+  P.epart = ep;
+  P.n_epart = c->n_rows;
+  P.ctl = ctl;
+  P.hist = hist;
+  P.hist_cap = 4096;
+  P.mlp_tol = opt->mlp_tolerance;
+  if (c->phys == 2) {
+    if ((rc = mlp_alloc(c->mlp, c->err))) return rc;
+    P.mlp = c->mlp.d_params;
+    P.mlp_stats = c->mlp.d_stats;
+  }
+  // reference positions on the device (stress output)
+  if ((rc = put_nodes(c, c->d_X0, c->X0u.data()))) return rc;
+  return reset_state(c);
+}
+
+extern "C" int cpi_create(const cpi_mesh_desc *m, const cpi_material *mat, const cpi_wall *wall,
+                          const cpi_options *opt, cpi_ctx **out) {
+  if (!out) {
+    set_global_error("cpi_create: out is NULL");
+    return CPI_ERR_INVALID;
+  }
+  *out = nullptr;
+  if (!m || !mat) {
+    set_global_error("cpi_create: mesh and material are required");
+    return CPI_ERR_INVALID;
+  }
+  cpi_options def;
+  cpi_default_options(&def);
+  cpi_ctx *c = new (std::nothrow) cpi_ctx;
+  if (!c) {
+    set_global_error("cpi_create: out of host memory");
+    return CPI_ERR_ALLOC;
+  }
+  int rc;
+  try {
+    rc = create_impl(c, m, mat, wall, opt ? opt : &def);
+  } catch (const std::bad_alloc &) {
+    c->err = "cpi_create: out of host memory";
+    rc = CPI_ERR_ALLOC;
+  }
+  if (rc) {
+    set_global_error(c->err);
+    cpi_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return CPI_OK;
+}
+
+extern "C" int cpi_set_state(cpi_ctx *c, const double *x0, const double *v0) {
+  if (!c) return CPI_ERR_INVALID;
+  if (x0) {
+    for (long i = 0; i < 3L * c->nn; i++)
+      if (!std::isfinite(x0[i])) return fail(c, CPI_ERR_INVALID, "non-finite position");
+    c->X0u.assign(x0, x0 + 3L * c->nn);
+    c->mass_ready = false;
+  }
+  if (v0) {
+    for (long i = 0; i < 3L * c->nn; i++)
+      if (!std::isfinite(v0[i])) return fail(c, CPI_ERR_INVALID, "non-finite velocity");
+    c->V0u.assign(v0, v0 + 3L * c->nn);
+  }
+  c->inited = false;
+  drop_graphs(c);
+  int rc = put_nodes(c, c->d_X0, c->X0u.data());
+  if (rc) return rc;
+  return reset_state(c);
+}
+
+extern "C" int cpi_stable_dt(cpi_ctx *c, double *dte, double *dtc) {
+  if (!c || !dte || !dtc) return CPI_ERR_INVALID;
+  if (!c->inited) {
+    int rc = reset_state(c);
+    if (!rc && !c->mass_ready) rc = compute_mass(c);
+    if (rc) return rc;
+  }
+  return stable_dt(c, dte, dtc);
+}
+
+extern "C" int cpi_init(cpi_ctx *c, double dt) {
+  if (!c) return CPI_ERR_INVALID;
+  if (!std::isfinite(dt)) return fail(c, CPI_ERR_INVALID, "cpi_init: dt must be finite");
+  drop_graphs(c);
+  c->inited = false;
+  int rc = reset_state(c);
+  if (!rc) rc = compute_mass(c);
+  if (rc) return rc;
+  if (dt <= 0.0) {
+    double dte, dtc;
+    if ((rc = stable_dt(c, &dte, &dtc))) return rc;
+    const double ds = 1.0 / std::sqrt(1.0 / (dte * dte) + 1.0 / (dtc * dtc));
+    dt = (dt == 0.0 ? 0.5 : -dt) * ds;
+    if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, CPI_ERR_NUMERIC, "stable dt is not positive");
+  }
+  c->dt = dt;
+  c->step = 0;
+  Ctl h{};
+  h.dt = dt;
+  HIPCHK(c, hipMemcpyAsync(c->P.ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+  hipError_t e = launch_node_init(c->P, c->stream);
+  if (e == hipSuccess) e = launch_energy_reduce(c->P, c->stream);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("init kernels: ") + hipGetErrorString(e));
+  HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->opt.use_graph) {
+    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux))) return rc;
+  }
+  c->inited = true;
+  return CPI_OK;
+}
+
+extern "C" int cpi_step(cpi_ctx *c, int32_t nsteps) {
+  if (!c) return CPI_ERR_INVALID;
+  if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_step before cpi_init");
+  if (nsteps < 0) return fail(c, CPI_ERR_INVALID, "nsteps < 0");
+  for (int i = 0; i < nsteps; i++) {
+    const bool aux = (i == nsteps - 1);
+    if (c->g_plain) {
+      HIPCHK(c, hipGraphLaunch(aux ? c->g_aux : c->g_plain, c->stream));
+    } else {
+      int rc = enqueue_step(c, aux);
+      if (rc) return rc;
+    }
+  }
+  c->step += nsteps;
+  return CPI_OK;
+}
+
+extern "C" int cpi_synchronize(cpi_ctx *c) {
+  if (!c) return CPI_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+extern "C" int cpi_get_state(cpi_ctx *c, double *x, double *v, double *a, double *m, double *vi) {
+  if (!c) return CPI_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = CPI_OK;
+  if (x && (rc = get_nodes(c, c->P.x, x))) return rc;
+  std::vector<double> hv, ha;
+  if (v || vi) {
+    hv.resize(3L * c->nn);
+    if ((rc = get_nodes(c, c->P.v, hv.data()))) return rc;
+    if (v) std::memcpy(v, hv.data(), sizeof(double) * hv.size());
+  }
+  if (a || vi) {
+    ha.resize(3L * c->nn);
+    if ((rc = get_nodes(c, c->P.acc, ha.data()))) return rc;
+    if (a) std::memcpy(a, ha.data(), sizeof(double) * ha.size());
+  }
+  if (vi)
+    for (long i = 0; i < 3L * c->nn; i++) vi[i] = hv[i] - 0.5 * c->dt * ha[i];
+  if (m) {
+    std::vector<double> hm(c->nn);
+    HIPCHK(c, hipMemcpy(hm.data(), c->P.m, sizeof(double) * c->nn, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->nn; i++) m[c->nperm[i]] = hm[i];
+  }
+  return c->inited ? check_flags(c) : CPI_OK;
+}
+
+extern "C" int cpi_get_element_state(cpi_ctx *c, double *sig, double *ep) {
+  if (!c) return CPI_ERR_INVALID;
+  if (c->phys == CPI_PHYSICS_COMPAT) {
+    if (sig) std::memset(sig, 0, sizeof(double) * 6 * c->nq * (size_t)c->ne);
+    if (ep) std::memset(ep, 0, sizeof(double) * c->nq * (size_t)c->ne);
+    return CPI_OK;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->nq * c->ne;
+  auto fetch = [&](const void *d, size_t cnt, std::vector<double> &out) -> int {
+    out.resize(cnt);
+    if (c->prec == CPI_PREC_F32) {
+      std::vector<float> t(cnt);
+      HIPCHK(c, hipMemcpy(t.data(), d, 4 * cnt, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < cnt; i++) out[i] = t[i];
+    } else {
+      HIPCHK(c, hipMemcpy(out.data(), d, 8 * cnt, hipMemcpyDeviceToHost));
+    }
+    return CPI_OK;
+  };
+  std::vector<double> hs, he;
+  int rc;
+  if (sig) {
+    if ((rc = fetch(c->P.sig, 6 * n, hs))) return rc;
+    for (int e = 0; e < c->ne; e++)
+      for (int q = 0; q < c->nq; q++)
+        for (int k = 0; k < 6; k++)
+          sig[((size_t)c->eperm[e] * c->nq + q) * 6 + k] = hs[(size_t)(q * 6 + k) * c->ne + e];
+  }
+  if (ep) {
+    if ((rc = fetch(c->P.epbar, n, he))) return rc;
+    for (int e = 0; e < c->ne; e++)
+      for (int q = 0; q < c->nq; q++) ep[(size_t)c->eperm[e] * c->nq + q] = he[(size_t)q * c->ne + e];
+  }
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_stress_strain(cpi_ctx *c, double *strain, double *stress) {
+  if (!c) return CPI_ERR_INVALID;
+  if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_get_stress_strain before cpi_init");
+  int rc = ensure_tmp(c, 12 * (size_t)c->ne);
+  if (rc) return rc;
+  const double *X0[3] = {c->d_X0[0], c->d_X0[1], c->d_X0[2]};
+  hipError_t e = launch_stress_elem(c->P, X0, c->d_tmp, c->stream);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
+  std::vector<double> h(12 * (size_t)c->ne), byuser(12 * (size_t)c->ne);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tmp, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int e2 = 0; e2 < c->ne; e2++)
+    std::memcpy(&byuser[12 * (size_t)c->eperm[e2]], &h[12 * (size_t)e2], 12 * sizeof(double));
+  // element order, last element containing the node wins (dynamics_kernels.h:141-147)
+  for (int e2 = 0; e2 < c->ne; e2++)
+    for (int j = 0; j < c->npe; j++) {
+      int n = c->conn_u[(size_t)c->npe * e2 + j];
+      for (int k = 0; k < 6; k++) {
+        if (strain) strain[6L * n + k] = byuser[12 * (size_t)e2 + k];
+        if (stress) stress[6L * n + k] = byuser[12 * (size_t)e2 + 6 + k];
+      }
+    }
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_energy(cpi_ctx *c, double *row, int64_t *step) {
+  if (!c || !row) return CPI_ERR_INVALID;
+  Ctl h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->P.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::memcpy(row, h.row, sizeof(double) * CPI_NENERGY);
+  if (step) *step = h.step;
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_energy_history(cpi_ctx *c, double *rows, int32_t cap, int32_t *count) {
+  if (!c || !count || cap < 0) return CPI_ERR_INVALID;
+  Ctl h;
+  HIPCHK(c, hipMemcpyAsync(&h, c->P.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int64_t have = std::min<int64_t>(h.nrows, c->P.hist_cap);
+  const int64_t n = std::min<int64_t>(have, cap);
+  std::vector<double> all(8 * (size_t)c->P.hist_cap);
+  HIPCHK(c, hipMemcpy(all.data(), c->P.hist, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n; i++) {
+    int64_t r = h.nrows - n + i;
+    std::memcpy(rows + 8 * i, &all[8 * (size_t)(r % c->P.hist_cap)], 8 * sizeof(double));
+  }
+  *count = (int32_t)n;
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_info(cpi_ctx *c, cpi_info *info) {
+  if (!c || !info) return CPI_ERR_INVALID;
+  std::memset(info, 0, sizeof(*info));
+  info->num_nodes = c->nn;
+  info->num_elements = c->ne;
+  info->nodes_per_element = c->npe;
+  info->num_quadrature_pts = c->nq;
+  info->num_colors = c->ncol;
+  info->num_blocks = c->scatter == CPI_SCATTER_BLOCK ? c->nblocks : 0;
+  info->num_boundary_nodes = c->scatter == CPI_SCATTER_BLOCK ? c->plan.n_boundary : c->nn;
+  info->num_partial_slots = c->scatter == CPI_SCATTER_BLOCK ? c->plan.n_slots : 0;
+  // algorithmic bytes: connectivity 4 npe and state 14 nqs s per element,
+  // then per node what each pass must touch once
+  const double s = c->prec == CPI_PREC_F32 ? 4 : 8;
+  const double state = c->phys == CPI_PHYSICS_COMPAT ? 0.0 : 14.0 * c->nq * s;
+  const double per_elem = 4.0 * c->npe + state;
+  info->bytes_per_step = c->ne * per_elem + 224.0 * c->nn;
+  if (c->scatter == CPI_SCATTER_BLOCK) {
+    const double nb = c->plan.n_boundary, ni = c->plan.n_interior;
+    info->elem_kernel_bytes = c->ne * per_elem + ni * 104.0 + nb * 72.0;
+    info->node_kernel_bytes = nb * 128.0;
+  } else {
+    info->elem_kernel_bytes = c->ne * per_elem + c->nn * 72.0;
+    info->node_kernel_bytes = c->nn * 152.0;
+  }
+  info->dt = c->dt;
+  info->step = c->step;
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_coloring(cpi_ctx *c, int32_t *color, int32_t *ncol) {
+  if (!c) return CPI_ERR_INVALID;
+  if (color) std::memcpy(color, c->color.data(), sizeof(int32_t) * c->ne);
+  if (ncol) *ncol = c->ncol;
+  return CPI_OK;
+}
+
+extern "C" int cpi_time_steps(cpi_ctx *c, int32_t nsteps, double *ms) {
+  if (!c || !ms) return CPI_ERR_INVALID;
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  HIPCHK(c, hipEventRecord(a, c->stream));
+  int rc = cpi_step(c, nsteps);
+  HIPCHK(c, hipEventRecord(b, c->stream));
+  HIPCHK(c, hipEventSynchronize(b));
+  float t = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = t;
+  return rc;
+}
+
+extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *out, int32_t cap,
+                                   int32_t *count) {
+  if (!c || !out || !count) return CPI_ERR_INVALID;
+  if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_profile_kernels before cpi_init");
+  const char *names[2] = {c->scatter == CPI_SCATTER_BLOCK ? "elem_block" : "elem_direct",
+                          c->scatter == CPI_SCATTER_BLOCK ? "node_bnd" : "node_all"};
+  double tot[2] = {0, 0};
+  int64_t launches[2] = {0, 0};
+  hipEvent_t ev[3];
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  for (int i = 0; i < nsteps; i++) {
+    Params p = c->P;
+    p.store_aux = 0;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    hipError_t e = hipSuccess;
+    if (c->scatter == CPI_SCATTER_BLOCK) {
+      p.epart_base = 0;
+      e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream);
+      launches[0] += 1;
+    } else {
+      const bool atomic = c->scatter == CPI_SCATTER_ATOMIC;
+      const int nc = atomic ? 1 : c->ncol;
+      for (int k = 0; k < nc && e == hipSuccess; k++) {
+        p.elem_base = atomic ? 0 : c->color_off[k];
+        p.elem_count = atomic ? c->ne : c->color_off[k + 1] - c->color_off[k];
+        p.epart_base = c->color_rows[k];
+        e = launch_elem_direct(p, c->rule, c->prec, c->phys, atomic, c->stream);
+        launches[0] += 1;
+      }
+    }
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    p.epart_base = c->n_elem_rows;
+    if (e == hipSuccess)
+      e = c->scatter == CPI_SCATTER_BLOCK ? launch_node_bnd(p, false, c->stream)
+                                          : launch_node_all(p, false, c->stream);
+    launches[1] += 1;
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
+    HIPCHK(c, hipEventSynchronize(ev[2]));
+    float t0 = 0, t1 = 0;
+    HIPCHK(c, hipEventElapsedTime(&t0, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&t1, ev[1], ev[2]));
+    tot[0] += t0;
+    tot[1] += t1;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  c->step += nsteps;
+  int n = std::min<int>(cap, 2);
+  for (int k = 0; k < n; k++) {
+    std::memset(out[k].name, 0, sizeof(out[k].name));
+    std::strncpy(out[k].name, names[k], sizeof(out[k].name) - 1);
+    out[k].total_ms = tot[k];
+    out[k].launches = launches[k];
+  }
+  *count = n;
+  return CPI_OK;
+}
+
+extern "C" int cpi_debug_strain(cpi_ctx *c, double alpha, int32_t dcase, double *W, double *V) {
+  if (!c || !W || !V) return CPI_ERR_INVALID;
+  if (dcase != 0 && dcase != 1) return fail(c, CPI_ERR_INVALID, "deformation case must be 0 or 1");
+  const int nb = (c->ne + 255) / 256;
+  int rc = ensure_tmp(c, 2 * (size_t)nb);
+  if (rc) return rc;
+  Params p = c->P;
+  for (int k = 0; k < 3; k++) p.x[k] = c->d_X0[k];
+  hipError_t e = launch_debug_strain(p, c->rule, alpha, dcase, c->mat.nu, c->d_tmp, c->stream);
+  if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
+  std::vector<double> h(2 * (size_t)nb);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tmp, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double w = 0, v = 0;
+  for (int i = 0; i < nb; i++) {
+    w += h[2 * i];
+    v += h[2 * i + 1];
+  }
+  *W = w;
+  *V = v;
+  return CPI_OK;
+}

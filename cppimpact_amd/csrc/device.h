@@ -1,0 +1,169 @@
+// Device-side data structures and element/node physics shared by the
+// kernels of libcppimpact_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cpi {
+
+constexpr int kMaxQ = 5;
+
+// energy partial row: one per workgroup of the element and node kernels
+enum : int {
+  EP_WINT = 0,  // cumulative internal work
+  EP_WPL,       // cumulative plastic dissipation
+  EP_MINDT,     // stable dt of this step's configuration (min)
+  EP_BAD,       // cumulative count of elements with detJ <= 0
+  EP_KE,        // kinetic energy of the nodes updated by the workgroup
+  EP_PEN,       // penalty energy
+  EP_FC,        // contact force
+  EP_WGRAV,     // cumulative gravity work
+  EP_N
+};
+
+struct Ctl {
+  double dt;
+  int64_t step;    // completed loop iterations
+  int64_t nrows;   // energy rows written to the history ring
+  uint32_t flags;  // bit 0: non-finite energy, bit 1: inverted element
+  uint32_t pad;
+  double e0;       // KE + Epen of the initial state
+  double row[8];   // latest energy row (CPI_E_* order)
+};
+
+struct MlpParams;  // surrogate, see mlp.h
+
+struct Params {
+  int nn, ne, npe, nq;
+  // material (D per tetrahedral.h:486-512: c11 = (1-nu) k, c12 = nu k,
+  // c44 = (1-2nu)/2 k with k = E/((1+nu)(1-2nu)))
+  double c11, c12, c44, mu, sy0, H, wave_speed;
+  // wall (wall.h:37-44)
+  int wall_on, wall_dim, wall_normal, pad0;
+  double wall_loc, wall_k, wall_nk;
+  const uint8_t *contact_mask;  // slave-only contact, or null for all nodes
+  double grav[3];
+  // quadrature points (tet10) and the weight sum (tet4)
+  double qw[kMaxQ];
+  double qpt[kMaxQ][3];
+  double wsum;
+  // nodal state, device node order, structure of arrays
+  double *x[3];
+  double *v[3];
+  const double *m;
+  double *f[3];          // assembled force (colour / atomic scatter)
+  double *acc[3];        // a_n            written when store_aux
+  double *du[3];         // du of the step written when store_aux
+  double *xprev[3];      // x_n before the step, written when store_aux
+  const int32_t *conn;   // [npe][ne] device node ids (colour / atomic)
+  void *sig;             // [nq][6][ne] stress state (ep)
+  void *epbar;           // [nq][ne]
+  // workgroup plan (LDS-assembled scatter)
+  const int32_t *meta;
+  const uint16_t *conn_local;
+  const uint16_t *csr_ptr;
+  const uint16_t *csr_idx;
+  const int32_t *bnd_nodes;
+  const int32_t *bnd_slot;
+  const int32_t *slot_ptr;
+  double *partial;       // [slots][3]
+  int n_interior, n_boundary, max_loc, max_el;
+  // energy partial rows and control block
+  double *epart;
+  int epart_base;        // first row of this launch
+  int n_epart;
+  Ctl *ctl;
+  double *hist;          // [hist_cap][8]
+  int hist_cap;
+  int store_aux;
+  int elem_base;         // colour / atomic: first device element of launch
+  int elem_count;
+  const MlpParams *mlp;  // null unless the surrogate is enabled
+  double mlp_tol;        // fallback threshold on |f_yield| / sy0
+  int *mlp_stats;        // [2] surrogate accepted / fell back (optional)
+  float *samples;        // return-mapping sample buffer (optional)
+  int *sample_count;
+  int sample_cap;
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+// hardware dispatches workgroups round-robin over the 8 XCDs; give each XCD
+// a contiguous range of logical blocks so neighbouring blocks, which share
+// nodes, hit the same L2
+__device__ __forceinline__ int xcd_remap(int hw, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int xcd = hw & 7, k = hw >> 3;
+  return xcd * q + min(xcd, r) + k;
+}
+
+template <typename R>
+__device__ __forceinline__ R inv3(const R J[9], R Ji[9]) {
+  R c0 = J[4] * J[8] - J[5] * J[7];
+  R c1 = J[5] * J[6] - J[3] * J[8];
+  R c2 = J[3] * J[7] - J[4] * J[6];
+  R det = J[0] * c0 + J[1] * c1 + J[2] * c2;
+  R r = R(1) / det;
+  Ji[0] = c0 * r;
+  Ji[1] = (J[2] * J[7] - J[1] * J[8]) * r;
+  Ji[2] = (J[1] * J[5] - J[2] * J[4]) * r;
+  Ji[3] = c1 * r;
+  Ji[4] = (J[0] * J[8] - J[2] * J[6]) * r;
+  Ji[5] = (J[2] * J[3] - J[0] * J[5]) * r;
+  Ji[6] = c2 * r;
+  Ji[7] = (J[1] * J[6] - J[0] * J[7]) * r;
+  Ji[8] = (J[0] * J[4] - J[1] * J[3]) * r;
+  return det;
+}
+
+// quadratic basis gradients at (x,y,z), tetrahedral.h:199-250 layout
+__device__ __forceinline__ void tet10_grad(double x, double y, double z, double G[30]) {
+  G[0] = 4 * x - 1; G[1] = 0.0; G[2] = 0.0;
+  G[3] = 0.0; G[4] = 4 * y - 1; G[5] = 0.0;
+  G[6] = 4 * x + 4 * y + 4 * z - 3;
+  G[7] = 4 * y + 4 * x + 4 * z - 3;
+  G[8] = 4 * z + 4 * x + 4 * y - 3;
+  G[9] = 0.0; G[10] = 0.0; G[11] = 4.0 * z - 1.0;
+  G[12] = 4 * y; G[13] = 4 * x; G[14] = 0.0;
+  G[15] = -4 * y; G[16] = -8 * y - 4 * x - 4 * z + 4; G[17] = -4 * y;
+  G[18] = -8 * x - 4 * y - 4 * z + 4; G[19] = -4 * x; G[20] = -4 * x;
+  G[21] = 4 * z; G[22] = 0.0; G[23] = 4 * x;
+  G[24] = 0.0; G[25] = 4 * z; G[26] = 4 * y;
+  G[27] = -4 * z; G[28] = -4 * z; G[29] = -8 * z - 4 * x - 4 * y + 4;
+}
+
+// energies accumulated by one thread
+struct EAcc {
+  double wint = 0.0, wpl = 0.0, mindt = 1e300, bad = 0.0;
+  double ke = 0.0, pen = 0.0, fc = 0.0, wgrav = 0.0;
+};
+
+// J2 return map, linear isotropic hardening (Simo & Hughes 1998 Box 3.2
+// restricted to isotropic hardening); s holds the trial stress on entry.
+template <typename R>
+__device__ __forceinline__ R radial_return(const Params &p, R s[6], R &ep) {
+  const R third = R(1) / R(3);
+  R pm = (s[0] + s[1] + s[2]) * third;
+  R d0 = s[0] - pm, d1 = s[1] - pm, d2 = s[2] - pm;
+  R nrm = sqrt(d0 * d0 + d1 * d1 + d2 * d2 +
+               R(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+  const R k23 = R(0.81649658092772603273);  // sqrt(2/3)
+  R fy = nrm - k23 * (R(p.sy0) + R(p.H) * ep);
+  if (!(fy > R(0))) return R(0);
+  R dg = fy / (R(2) * R(p.mu) + R(2.0 / 3.0) * R(p.H));
+  R sc = R(2) * R(p.mu) * dg / nrm;
+  s[0] -= sc * d0;
+  s[1] -= sc * d1;
+  s[2] -= sc * d2;
+  s[3] -= sc * s[3];
+  s[4] -= sc * s[4];
+  s[5] -= sc * s[5];
+  ep += k23 * dg;
+  return dg;
+}
+
+}  // namespace cpi

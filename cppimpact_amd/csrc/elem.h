@@ -1,0 +1,338 @@
+// Element physics on the device: tet4 and tet10 internal force, stress
+// update, lumped mass and the stable-dt estimate.  Included by kernels.hip.
+#pragma once
+
+#include "device.h"
+#include "mlp.h"
+
+namespace cpi {
+
+enum : int { PHYS_COMPAT = 0, PHYS_EP = 1, PHYS_EP_MLP = 2 };
+enum : int { RULE_REF5 = 0, RULE_REF4 = 1, RULE_EXACT4 = 2, RULE_CENTROID = 3 };
+
+// Quadrature rules as compile-time constants so that the tet10 basis
+// gradients at the points fold into immediates.
+template <int RULE> struct Quad;
+template <> struct Quad<RULE_REF5> {  // tetrahedral.h:12-40
+  static constexpr int n = 5;
+  __device__ static constexpr double w(int q) { return q == 0 ? -0.13333333333 : 0.075; }
+  __device__ static constexpr double pt(int q, int c) {
+    return q == 0 ? 0.25 : (q - 1 == c ? 0.5 : 0.16666666666);
+  }
+};
+template <> struct Quad<RULE_REF4> {  // tetrahedral.h:48-73
+  static constexpr int n = 4;
+  __device__ static constexpr double w(int) { return 0.041666667; }
+  __device__ static constexpr double pt(int q, int c) {
+    return (q >= 1 && q - 1 == c) ? 0.58541020 : 0.13819660;
+  }
+};
+template <> struct Quad<RULE_EXACT4> {  // alpha = (5+3 sqrt5)/20, beta = (5-sqrt5)/20
+  static constexpr int n = 4;
+  __device__ static constexpr double w(int) { return 1.0 / 24.0; }
+  __device__ static constexpr double pt(int q, int c) {
+    return (q >= 1 && q - 1 == c) ? 0.58541019662496845446 : 0.13819660112501051518;
+  }
+};
+template <> struct Quad<RULE_CENTROID> {
+  static constexpr int n = 1;
+  __device__ static constexpr double w(int) { return 1.0 / 6.0; }
+  __device__ static constexpr double pt(int, int) { return 0.25; }
+};
+
+// number of stress-state points an element carries
+template <int NPE, int RULE> struct NState {
+  static constexpr int n = NPE == 4 ? 1 : Quad<RULE>::n;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_state(const Params &p, int e, int q, R s[6], R &ep) {
+  const R *S = (const R *)p.sig;
+  const R *P = (const R *)p.epbar;
+  const long ne = p.ne;
+#pragma unroll
+  for (int c = 0; c < 6; c++) s[c] = S[(long)(q * 6 + c) * ne + e];
+  ep = P[(long)q * ne + e];
+}
+
+template <typename R>
+__device__ __forceinline__ void store_state(const Params &p, int e, int q, const R s[6], R ep) {
+  R *S = (R *)p.sig;
+  R *P = (R *)p.epbar;
+  const long ne = p.ne;
+#pragma unroll
+  for (int c = 0; c < 6; c++) S[(long)(q * 6 + c) * ne + e] = s[c];
+  P[(long)q * ne + e] = ep;
+}
+
+// Stress at one point from the strain increment de.  compat: sigma = D de
+// with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
+// sigma_n + D de, then the return map (exact or surrogate).  Adds the
+// point's internal work and plastic dissipation (times wd) to ea.
+template <typename R, int PHYS>
+__device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
+                                             R wd, R s[6], EAcc &ea) {
+  const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
+  R ds[6];
+  ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
+  ds[1] = c11 * de[1] + c12 * (de[0] + de[2]);
+  ds[2] = c11 * de[2] + c12 * (de[0] + de[1]);
+  ds[3] = c44 * de[3];
+  ds[4] = c44 * de[4];
+  ds[5] = c44 * de[5];
+  if constexpr (PHYS == PHYS_COMPAT) {
+    R w = R(0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      s[c] = ds[c];
+      w += s[c] * de[c];
+    }
+    ea.wint += double(wd * w);
+  } else {
+    R so[6], ep;
+    load_state<R>(p, e, q, so, ep);
+#pragma unroll
+    for (int c = 0; c < 6; c++) s[c] = so[c] + ds[c];
+    R dg;
+    if constexpr (PHYS == PHYS_EP_MLP) {
+      dg = mlp_return<R>(p, de, so, ep, s);
+    } else {
+      R ep0 = ep;
+      dg = radial_return<R>(p, s, ep);
+      if (p.samples && dg > R(0)) record_sample<R>(p, de, so, ep0, s, ep);
+    }
+    R w = R(0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) w += (so[c] + s[c]) * de[c];
+    ea.wint += double(wd * R(0.5) * w);
+    if (dg > R(0))
+      ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
+    store_state<R>(p, e, q, s, ep);
+  }
+}
+
+// f_i = wd * sigma . dN_i
+template <typename R>
+__device__ __forceinline__ void add_bts(R f[3], const R dn[3], const R s[6], R wd) {
+  f[0] += wd * (dn[0] * s[0] + dn[1] * s[3] + dn[2] * s[5]);
+  f[1] += wd * (dn[1] * s[1] + dn[0] * s[3] + dn[2] * s[4]);
+  f[2] += wd * (dn[2] * s[2] + dn[1] * s[4] + dn[0] * s[5]);
+}
+
+// minimum altitude of the corner tetrahedron / wave speed
+template <typename R>
+__device__ __forceinline__ double corner_dt(const R e1[3], const R e2[3], const R e3[3], R vol6,
+                                            double c, double factor) {
+  auto cr2 = [](const R a[3], const R b[3]) {
+    R x = a[1] * b[2] - a[2] * b[1], y = a[2] * b[0] - a[0] * b[2], z = a[0] * b[1] - a[1] * b[0];
+    return x * x + y * y + z * z;
+  };
+  R d21[3] = {e2[0] - e1[0], e2[1] - e1[1], e2[2] - e1[2]};
+  R d31[3] = {e3[0] - e1[0], e3[1] - e1[1], e3[2] - e1[2]};
+  R a = cr2(d21, d31);
+  a = max(a, cr2(e2, e3));
+  a = max(a, cr2(e1, e3));
+  a = max(a, cr2(e1, e2));
+  return double(fabs(vol6)) / sqrt(double(a)) / (c * factor);
+}
+
+constexpr double kTet10LengthFactor = 4.35;
+
+// ---------------------------------------------------------------------------
+// tet4: B is constant, every quadrature point carries the same strain, so
+// the rule enters only through its weight sum (compat) and the element keeps
+// one stress point (ep).
+// ---------------------------------------------------------------------------
+template <typename R, int PHYS>
+__device__ __forceinline__ void tet4_force(const Params &p, const double (&xs)[4][3],
+                                           const double (&vs)[4][3], double dt, int e,
+                                           R (&f)[4][3], EAcc &ea) {
+  R J[9];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) J[a * 3 + q] = R(xs[q + 1][a] - xs[0][a]);
+  R Ji[9];
+  R detJ = inv3<R>(J, Ji);
+  if (!(detJ > R(0))) ea.bad += 1.0;
+  R u[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) u[k][c] = R(dt * vs[k + 1][c] - dt * vs[0][c]);
+  // L[i][j] = d du_i / d x_j
+  R L[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      L[i][j] = Ji[0 * 3 + j] * u[0][i] + Ji[1 * 3 + j] * u[1][i] + Ji[2 * 3 + j] * u[2][i];
+  R de[6] = {L[0][0], L[1][1], L[2][2], L[0][1] + L[1][0], L[1][2] + L[2][1],
+             L[0][2] + L[2][0]};
+  R wd = R(p.wsum) * detJ;
+  R s[6];
+  point_stress<R, PHYS>(p, e, 0, de, wd, s, ea);
+#pragma unroll
+  for (int c = 0; c < 3; c++) f[0][c] = R(0);
+#pragma unroll
+  for (int k = 1; k < 4; k++) {
+    R dn[3] = {Ji[(k - 1) * 3 + 0], Ji[(k - 1) * 3 + 1], Ji[(k - 1) * 3 + 2]};
+    f[k][0] = f[k][1] = f[k][2] = R(0);
+    add_bts<R>(f[k], dn, s, wd);
+#pragma unroll
+    for (int c = 0; c < 3; c++) f[0][c] -= f[k][c];
+  }
+  if constexpr (PHYS != PHYS_COMPAT) {
+    R e1[3] = {J[0], J[3], J[6]}, e2[3] = {J[1], J[4], J[7]}, e3[3] = {J[2], J[5], J[8]};
+    double d = corner_dt<R>(e1, e2, e3, detJ, p.wave_speed, 1.0);
+    ea.mindt = fmin(ea.mindt, d);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tet10: loop over the rule's points; coordinates relative to corner 2 (the
+// origin of the reference's quadratic parametrisation)
+// ---------------------------------------------------------------------------
+template <int RULE, typename R, int PHYS>
+__device__ __forceinline__ void tet10_force(const Params &p, const double (&xs)[10][3],
+                                            const double (&vs)[10][3], double dt, int e,
+                                            R (&f)[10][3], EAcc &ea) {
+  R xr[10][3], ur[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; i++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      xr[i][c] = R(xs[i][c] - xs[2][c]);
+      ur[i][c] = R(dt * vs[i][c]);
+      f[i][c] = R(0);
+    }
+  using Q = Quad<RULE>;
+#pragma unroll
+  for (int q = 0; q < Q::n; q++) {
+    double Gd[30];
+    tet10_grad(Q::pt(q, 0), Q::pt(q, 1), Q::pt(q, 2), Gd);
+    R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * xr[i][a];
+    R Ji[9];
+    R detJ = inv3<R>(J, Ji);
+    if (!(detJ > R(0))) ea.bad += 1.0;
+    R de[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+      R dn[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
+                Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+      de[0] += dn[0] * ur[i][0];
+      de[1] += dn[1] * ur[i][1];
+      de[2] += dn[2] * ur[i][2];
+      de[3] += dn[1] * ur[i][0] + dn[0] * ur[i][1];
+      de[4] += dn[2] * ur[i][1] + dn[1] * ur[i][2];
+      de[5] += dn[2] * ur[i][0] + dn[0] * ur[i][2];
+    }
+    R wd = R(Q::w(q)) * detJ;
+    R s[6];
+    point_stress<R, PHYS>(p, e, q, de, wd, s, ea);
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+      R dn[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
+                Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+      add_bts<R>(f[i], dn, s, wd);
+    }
+  }
+  if constexpr (PHYS != PHYS_COMPAT) {
+    // corners in the linear ordering: (2, 0, 1, 3) is an even permutation
+    R e1[3], e2[3], e3[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      e1[c] = xr[0][c];
+      e2[c] = xr[1][c];
+      e3[c] = xr[3][c];
+    }
+    R v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+           e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, e3, v6, p.wave_speed, kTet10LengthFactor));
+  }
+}
+
+template <int NPE, int RULE, typename R, int PHYS>
+__device__ __forceinline__ void elem_force(const Params &p, const double (&xs)[NPE][3],
+                                           const double (&vs)[NPE][3], double dt, int e,
+                                           R (&f)[NPE][3], EAcc &ea) {
+  if constexpr (NPE == 4)
+    tet4_force<R, PHYS>(p, xs, vs, dt, e, f, ea);
+  else
+    tet10_force<RULE, R, PHYS>(p, xs, vs, dt, e, f, ea);
+}
+
+// ---------------------------------------------------------------------------
+// lumped mass (FP64): ROWSUM restates analysis.h:136-162 with the rule's
+// points; HRZ gives rho V/4 (tet4), rho V/36 and 4 rho V/27 (tet10)
+// ---------------------------------------------------------------------------
+template <int NPE, int RULE>
+__device__ __forceinline__ void elem_mass(const Params &p, const double (&xs)[NPE][3], int kind,
+                                          double rho, double (&me)[NPE]) {
+  double e1[3], e2[3], e3[3];
+  const int o = NPE == 4 ? 0 : 2, i1 = NPE == 4 ? 1 : 0, i2 = NPE == 4 ? 2 : 1;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    e1[c] = xs[i1][c] - xs[o][c];
+    e2[c] = xs[i2][c] - xs[o][c];
+    e3[c] = xs[3][c] - xs[o][c];
+  }
+  double v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+              e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+  if (kind == 1) {
+    double M = rho * v6 / 6.0;
+    if constexpr (NPE == 4) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) me[i] = 0.25 * M;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) me[i] = M / 36.0;
+#pragma unroll
+      for (int i = 4; i < 10; i++) me[i] = 4.0 * M / 27.0;
+    }
+    return;
+  }
+  // row sum; straight-edged elements have a constant Jacobian, det = v6
+  using Q = Quad<RULE>;
+#pragma unroll
+  for (int i = 0; i < NPE; i++) me[i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q::n; q++) {
+    const double x = Q::pt(q, 0), y = Q::pt(q, 1), z = Q::pt(q, 2);
+    double N[NPE];
+    if constexpr (NPE == 4) {
+      N[0] = 1.0 - x - y - z;
+      N[1] = x;
+      N[2] = y;
+      N[3] = z;
+    } else {
+      double L1 = 1.0 - x - y - z, L2 = x, L3 = y, L4 = z;
+      N[0] = (2 * L2 - 1) * L2;
+      N[1] = (2 * L3 - 1) * L3;
+      N[2] = (2 * L1 - 1) * L1;
+      N[3] = (2 * L4 - 1) * L4;
+      N[4] = 4 * L2 * L3;
+      N[5] = 4 * L1 * L3;
+      N[6] = 4 * L1 * L2;
+      N[7] = 4 * L2 * L4;
+      N[8] = 4 * L3 * L4;
+      N[9] = 4 * L1 * L4;
+    }
+#pragma unroll
+    for (int i = 0; i < NPE; i++) me[i] += N[i] * Q::w(q) * v6 * rho;
+  }
+}
+
+}  // namespace cpi

@@ -1,0 +1,59 @@
+// Internal declarations shared by the host-side C++ and the HIP translation
+// unit of libcppimpact_hip.so.  Not part of the public ABI.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cpi {
+
+// Greedy colouring in element order, smallest free colour.  Returns the
+// number of colours.
+int greedy_coloring(int ne, int npe, const int32_t *conn, int nn,
+                    int32_t *color);
+
+// Workgroup partition for the LDS-assembled scatter.
+//
+// Elements are sorted along a 3-D Hilbert curve of their centroids and cut
+// into blocks of at most max_elems elements and max_nodes distinct nodes.
+// A node referenced by one block only is "interior" and is updated by that
+// block; the others are "boundary" nodes whose force arrives as one partial
+// per referencing block and is summed by the boundary-node kernel.
+//
+// Device node order: interior nodes of block 0, of block 1, ..., then the
+// boundary nodes ordered by (first referencing block, user id).
+// Device element order: block order, Hilbert order inside a block.
+struct BlockPlan {
+  int nblocks = 0;
+  int max_local_nodes = 0;  // over all blocks
+  int max_block_elems = 0;
+  int n_interior = 0;       // device ids [0, n_interior) are interior nodes
+  int n_boundary = 0;       // device ids [n_interior, n_interior+n_boundary)
+  int n_slots = 0;
+  std::vector<int32_t> elem_perm;  // device element -> user element
+  std::vector<int32_t> node_perm;  // device node -> user node
+  std::vector<int32_t> node_inv;   // user node -> device node
+  // per block, 8 int32: elem_start, n_elem, int_start, n_int, bnd_off,
+  // n_bnd, csr_off, loc_off (offset of the block's n_loc+1 csr offsets)
+  std::vector<int32_t> meta;
+  std::vector<uint16_t> conn_local;  // [E][npe] local node index
+  std::vector<int32_t> bnd_nodes;    // per block boundary nodes (device id)
+  std::vector<int32_t> bnd_slot;     // parallel to bnd_nodes: partial slot
+  std::vector<uint16_t> csr_ptr;     // per block n_loc+1 offsets
+  std::vector<uint16_t> csr_idx;     // contribution = elem_local*npe + corner
+  std::vector<int32_t> slot_ptr;     // per boundary node, n_boundary+1
+};
+
+bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
+                      const double *xloc, int max_elems, int max_nodes,
+                      BlockPlan &plan, std::string &err);
+
+// Element order sorted by colour (stable), and colour offsets.
+void color_order(int ne, const int32_t *color, int ncol,
+                 std::vector<int32_t> &perm, std::vector<int32_t> &offsets);
+
+// Hilbert index of a point with 21-bit integer coordinates.
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z);
+
+}  // namespace cpi

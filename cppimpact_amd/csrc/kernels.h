@@ -1,0 +1,30 @@
+// Launchers of the kernels in kernels.hip, called by the context code.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace cpi {
+
+size_t elem_block_smem(int npe, int prec, int max_loc, int max_el);
+int elem_block_threads(int npe);
+
+hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,
+                             int nblocks, size_t smem, hipStream_t s);
+hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem);
+hipError_t launch_elem_direct(const Params &p, int rule, int prec, int phys, bool atomic,
+                              hipStream_t s);
+hipError_t launch_node_bnd(const Params &p, bool aux, hipStream_t s);
+hipError_t launch_node_all(const Params &p, bool aux, hipStream_t s);
+hipError_t launch_node_init(const Params &p, hipStream_t s);
+hipError_t launch_energy_reduce(const Params &p, hipStream_t s);
+hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho, double *m,
+                       int nblocks, hipStream_t s);
+hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s);
+hipError_t launch_stress_elem(const Params &p, const double *const X0[3], double *out,
+                              hipStream_t s);
+hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcase, double nu,
+                               double *out, hipStream_t s);
+
+}  // namespace cpi

@@ -1,0 +1,304 @@
+/*
+ * cppimpact_hip.h -- C ABI of the MI355X (gfx950) explicit-dynamics step.
+ *
+ * The reference has no plugin or FFI layer: its drivers pick a backend at
+ * compile time (test.cu:14-18, FEA_debug.cu:18-22) between the CPU
+ * Dynamics (include/dynamics.h) and the CUDA Dynamics (include/dynamics.cuh),
+ * both exposing
+ *     Dynamics(Mesh*, BaseMaterial*, Wall*)          dynamics.h:64-83
+ *     initialize(T pos[3], T vel[3])                 dynamics.h:99-111
+ *     solve(double dt, double time_end, int export)  dynamics.h:414-512
+ *                                                     dynamics.cuh:352-525
+ * This library is what a third backend (CPPIMPACT_HIP_BACKEND) calls from
+ * host-only C++: every entry point takes plain pointers and sizes, returns
+ * 0 or a negative cpi_status, and records a message for cpi_last_error().
+ * The entry point each one replaces is named beside it.
+ *
+ * Host arrays use the reference's layouts: positions/velocities/accelerations
+ * are [num_nodes][3] doubles (Mesh::xloc, Dynamics::vel), connectivity is
+ * [num_elements][nodes_per_element] 0-based int32 (Mesh::element_nodes).
+ * The library keeps its own device layout (structure-of-arrays, elements and
+ * nodes permuted for locality) and converts on upload/download.
+ *
+ * Thread safety: one context must not be used from two threads at once.
+ */
+#ifndef CPPIMPACT_HIP_H
+#define CPPIMPACT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPI_ABI_VERSION 1
+
+typedef enum cpi_status {
+  CPI_OK = 0,
+  CPI_ERR_INVALID = -1,     /* bad argument or unsupported combination */
+  CPI_ERR_HIP = -2,         /* HIP runtime failure (gpuErrchk analogue) */
+  CPI_ERR_ALLOC = -3,       /* host or device allocation failed */
+  CPI_ERR_MESH = -4,        /* degenerate/inverted element, bad connectivity */
+  CPI_ERR_NUMERIC = -5,     /* NaN/Inf detected in the state */
+  CPI_ERR_IO = -6,          /* file could not be opened or parsed */
+  CPI_ERR_STATE = -7,       /* call made out of order (e.g. step before init) */
+  CPI_ERR_NO_DEVICE = -8    /* no usable gfx950 device */
+} cpi_status;
+
+/* physics of the element force */
+#define CPI_PHYSICS_COMPAT 0 /* f_e = K_e(x_n) du, no stress memory:
+                                analysis.h:321-382, dynamics_kernels.h:75-148 */
+#define CPI_PHYSICS_EP 1     /* f_e = sum_q w detJ B^T sigma, J2 radial return */
+
+/* arithmetic of the element kernels (nodal state, mass, dt and energies are
+   always FP64) */
+#define CPI_PREC_F64 0
+#define CPI_PREC_F32 1
+
+/* nodal force assembly */
+#define CPI_SCATTER_BLOCK 0  /* workgroup-local LDS assembly, deterministic */
+#define CPI_SCATTER_COLOR 1  /* greedy element colouring, one pass per colour */
+#define CPI_SCATTER_ATOMIC 2 /* FP64 global atomics (order not reproducible) */
+
+/* quadrature (tetrahedral.h:7-74) */
+#define CPI_QUAD_DEFAULT -1  /* compat: REF5; ep: CENTROID (tet4), EXACT4 (tet10) */
+#define CPI_QUAD_REF5 0      /* the reference's 5-point rule, truncated literals */
+#define CPI_QUAD_REF4 1      /* the reference's 4-point rule, truncated literals */
+#define CPI_QUAD_EXACT4 2    /* 4-point degree-2 rule, full precision */
+#define CPI_QUAD_CENTROID 3  /* 1 point */
+
+/* lumped mass */
+#define CPI_MASS_DEFAULT -1  /* compat: ROWSUM; ep: HRZ */
+#define CPI_MASS_ROWSUM 0    /* m_i = sum_q N_i w detJ rho (analysis.h:136-162) */
+#define CPI_MASS_HRZ 1       /* diagonal scaling: rho V/36, 4 rho V/27 (tet10) */
+
+/* stress update used by CPI_PHYSICS_EP */
+#define CPI_RETURN_EXACT 0   /* closed-form radial return */
+#define CPI_RETURN_MLP 1     /* MLP surrogate, exact return where its yield
+                                residual exceeds mlp_tolerance * Y0 */
+
+/* energy row layout (cpi_get_energy) */
+#define CPI_E_KINETIC 0   /* 1/2 sum m |v_n|^2 */
+#define CPI_E_INTERNAL 1  /* cumulative internal work */
+#define CPI_E_PLASTIC 2   /* cumulative plastic dissipation (part of INTERNAL) */
+#define CPI_E_PENALTY 3   /* 1/2 k d^2 summed over penetrating nodes */
+#define CPI_E_GRAVITY 4   /* cumulative work of gravity */
+#define CPI_E_CONTACTF 5  /* total wall contact force along the normal */
+#define CPI_E_BALANCE 6   /* KE + Wint + Epen - Wgrav - (KE0 + Epen0) */
+#define CPI_E_MINDT 7     /* stable dt of the configuration stepped from */
+#define CPI_NENERGY 8
+
+typedef struct cpi_ctx cpi_ctx;
+
+/* Mesh<T, npe> public arrays (mesh.h:14-18) */
+typedef struct cpi_mesh_desc {
+  int32_t num_nodes;
+  int32_t num_elements;
+  int32_t nodes_per_element;     /* 4 (C3D4) or 10 (C3D10 node order) */
+  const double *xloc;            /* [num_nodes][3] */
+  const int32_t *element_nodes;  /* [num_elements][nodes_per_element] */
+  int32_t num_slave_nodes;       /* Mesh::slave_nodes; used only when
+                                    cpi_options.contact_slave_only is set */
+  const int32_t *slave_nodes;
+} cpi_mesh_desc;
+
+/* BaseMaterial{E, rho, nu} + Elastoplastic{beta, H, Y0}
+   (basematerial.h:5-20, elastoplastic.h:9-22).  beta is accepted and not
+   used (the reference never reads it); hardening is linear isotropic. */
+typedef struct cpi_material {
+  double E, rho, nu, beta, H, Y0;
+} cpi_material;
+
+/* Wall<T, dim, Basis>(name, location, stiffness, slave, n_slave, normal)
+   (wall.h:13-44): axis-aligned rigid plane; penetration d < 0 adds
+   -d * normal * stiffness / m to the acceleration along dim. */
+typedef struct cpi_wall {
+  int32_t enabled;
+  int32_t dim;      /* 0, 1, 2 (the reference instantiates 2) */
+  int32_t normal;   /* +1 or -1 */
+  double location;
+  double stiffness;
+} cpi_wall;
+
+typedef struct cpi_options {
+  int32_t physics;          /* CPI_PHYSICS_* */
+  int32_t precision;        /* CPI_PREC_* */
+  int32_t scatter;          /* CPI_SCATTER_* */
+  int32_t quadrature;       /* CPI_QUAD_* */
+  int32_t mass;             /* CPI_MASS_* */
+  int32_t stress_update;    /* CPI_RETURN_* */
+  int32_t use_graph;        /* replay the step as a hipGraph (default 1) */
+  int32_t track_energy;     /* per-step energy rows (default 1) */
+  int32_t contact_slave_only; /* 0 = every node, as the reference does */
+  int32_t block_elements;   /* elements per workgroup for SCATTER_BLOCK, 0 = auto */
+  int32_t device;           /* HIP device ordinal */
+  int32_t reserved0;
+  double gravity[3];        /* default (0, 0, -9.81): dynamics_kernels.h:165-167 */
+  double mlp_tolerance;     /* default 1e-3 */
+} cpi_options;
+
+typedef struct cpi_info {
+  int32_t num_nodes, num_elements, nodes_per_element, num_quadrature_pts;
+  int32_t num_colors;       /* greedy colouring (always computed) */
+  int32_t num_blocks;       /* SCATTER_BLOCK workgroups */
+  int32_t num_boundary_nodes; /* nodes shared between workgroups */
+  int32_t num_partial_slots;
+  double bytes_per_step;    /* algorithmic HBM bytes of one step */
+  double elem_kernel_bytes; /* algorithmic bytes of the element kernel */
+  double node_kernel_bytes; /* algorithmic bytes of the node kernel */
+  double dt;                /* current time step */
+  int64_t step;             /* completed loop iterations since cpi_init */
+} cpi_info;
+
+typedef struct cpi_kernel_time {
+  char name[48];
+  double total_ms;
+  int64_t launches;
+} cpi_kernel_time;
+
+/* ---- context ----------------------------------------------------------- */
+
+void cpi_default_options(cpi_options *opt);
+
+/* Dynamics ctor + the device half of Dynamics<...>::allocate()
+   (dynamics.h:64-83, dynamics.cuh:285-335).  Copies the mesh, builds the
+   colouring and the workgroup partition, uploads everything. */
+int cpi_create(const cpi_mesh_desc *mesh, const cpi_material *mat,
+               const cpi_wall *wall, const cpi_options *opt, cpi_ctx **out);
+
+/* Dynamics::~Dynamics + deallocate() (dynamics.h:85-95, dynamics.cuh:337-350) */
+void cpi_destroy(cpi_ctx *ctx);
+
+/* Message of the last failure on ctx (or of the last failed cpi_create when
+   ctx is NULL).  Never NULL. */
+const char *cpi_last_error(const cpi_ctx *ctx);
+
+/* Dynamics::initialize (dynamics.h:99-111): reference positions X0 and
+   initial velocities, [num_nodes][3].  Either may be NULL (keep mesh xloc /
+   zero velocity). */
+int cpi_set_state(cpi_ctx *ctx, const double *x0, const double *v0);
+
+/* Stable time step of the configuration X0: element CFL limit and the wall
+   penalty spring limit 2 sqrt(m_min / k).  Needs the mass, so it assembles
+   it if cpi_init has not run. */
+int cpi_stable_dt(cpi_ctx *ctx, double *dt_element, double *dt_contact);
+
+/* First half of Dynamics::solve (dynamics.h:439-466): x = X0, mass assembly,
+   the initial update with du = 0, and the half-kick v += dt/2 a0.
+   dt <= 0 selects dt = -dt * dt_stable (e.g. -0.5 gives half the stable
+   step; dt == 0 gives 0.5 * dt_stable). */
+int cpi_init(cpi_ctx *ctx, double dt);
+
+/* nsteps iterations of the loop body of Dynamics::solve
+   (dynamics.h:471-511, dynamics.cuh:450-517): du = dt v; update(); x += du;
+   v += dt a; v_i = v - dt/2 a.  Asynchronous: returns after enqueueing. */
+int cpi_step(cpi_ctx *ctx, int32_t nsteps);
+
+/* Waits for queued work; reports CPI_ERR_NUMERIC if a NaN/Inf or an inverted
+   element was seen. */
+int cpi_synchronize(cpi_ctx *ctx);
+
+/* Host copies of Dynamics::global_xloc, vel (v_{n+1/2}), global_acc (a_n),
+   global_mass (one value per node) and vel_i (v_n).  Any pointer may be NULL. */
+int cpi_get_state(cpi_ctx *ctx, double *x, double *v, double *a, double *m,
+                  double *v_i);
+
+/* Quadrature-point state: sig [num_elements][nqp][6] (xx yy zz xy yz xz),
+   epbar [num_elements][nqp].  Either may be NULL. */
+int cpi_get_element_state(cpi_ctx *ctx, double *sig, double *epbar);
+
+/* Nodal strain/stress as the reference's update() leaves them
+   (dynamics_kernels.h:117-147): per element at xi = 0 from the total
+   displacement x_n + du - X0 evaluated with B(x_n), each node taking the
+   value of the highest-numbered element that contains it.  Computed for the
+   last completed step.  [num_nodes][6] each. */
+int cpi_get_stress_strain(cpi_ctx *ctx, double *strain, double *stress);
+
+/* Latest energy row (CPI_E_*) and the step it belongs to. */
+int cpi_get_energy(cpi_ctx *ctx, double *row, int64_t *step);
+
+/* Up to cap most recent rows, oldest first; *count receives the number. */
+int cpi_get_energy_history(cpi_ctx *ctx, double *rows, int32_t cap,
+                           int32_t *count);
+
+int cpi_get_info(cpi_ctx *ctx, cpi_info *info);
+
+/* Greedy colour of every element (user order) and the colour count. */
+int cpi_get_coloring(cpi_ctx *ctx, int32_t *color, int32_t *num_colors);
+
+/* Enqueues nsteps steps between two HIP events on the context's stream and
+   returns the elapsed GPU time. */
+int cpi_time_steps(cpi_ctx *ctx, int32_t nsteps, double *elapsed_ms);
+
+/* Runs nsteps steps launching each kernel separately with HIP events
+   around every launch on the context's stream; fills per-kernel totals. */
+int cpi_profile_kernels(cpi_ctx *ctx, int32_t nsteps, cpi_kernel_time *out,
+                        int32_t cap, int32_t *count);
+
+/* Dynamics::debug_strain on the GPU (dynamics.h:303-412): applies the
+   known-answer displacement field (case 0: alpha x (1,-nu,-nu); case 1:
+   alpha x^2/2 (1,-nu,-nu)) and returns total strain energy and volume with
+   the context's quadrature. */
+int cpi_debug_strain(cpi_ctx *ctx, double alpha, int32_t def_case,
+                     double *energy, double *volume);
+
+/* ---- host-only utilities (no GPU needed) -------------------------------- */
+
+/* Greedy colouring in element order, smallest free colour; returns the
+   number of colours (>= 1) or a negative status. */
+int cpi_greedy_coloring(int32_t num_elements, int32_t nodes_per_element,
+                        const int32_t *element_nodes, int32_t num_nodes,
+                        int32_t *color);
+
+/* Structured box of nx*ny*nz hexes, each split into 6 tets sharing the cell
+   diagonal (conforming).  Node ids run x fastest; tet10 mid-side nodes sit
+   at edge midpoints on the doubled lattice.  Interior nodes are jittered by
+   U(-jitter, jitter) * spacing per coordinate from SplitMix64(seed).
+   Call with xloc == NULL to get sizes only. */
+typedef struct cpi_box_spec {
+  int32_t nx, ny, nz;
+  int32_t quadratic;
+  double lx, ly, lz;
+  double origin[3];
+  double jitter;
+  uint64_t seed;
+} cpi_box_spec;
+int cpi_mesh_box(const cpi_box_spec *spec, int32_t *num_nodes,
+                 int32_t *num_elements, double *xloc, int32_t *element_nodes);
+
+/* Thin-walled cylinder about z: (theta, z, r) grid of nt x nz x nr cells,
+   periodic in theta, mid-surface radius R, length L, thickness t; cells split
+   like the box; tet10 mid-side nodes at straight-edge midpoints. */
+typedef struct cpi_cylinder_spec {
+  int32_t ntheta, nz, nr;
+  int32_t quadratic;
+  double radius, length, thickness;
+  double origin[3];
+} cpi_cylinder_spec;
+int cpi_mesh_cylinder(const cpi_cylinder_spec *spec, int32_t *num_nodes,
+                      int32_t *num_elements, double *xloc,
+                      int32_t *element_nodes);
+
+/* Mesh::load_mesh (mesh.h:57-267) for the .inp subset the reference reads:
+   *Node, *Element (nodes_per_element ids taken per line), *Nset.  Ids are
+   1-based and assumed contiguous, as in the reference.  Node sets are
+   returned in name order; slave/fixed sets are those whose name contains
+   "slave"/"fixed" (case-insensitive).  Two-phase: call with out arrays NULL
+   to get the counts.  Returns CPI_ERR_IO when the file cannot be opened. */
+typedef struct cpi_inp_counts {
+  int32_t num_nodes, num_elements, num_node_sets, num_set_indices;
+  int32_t num_fixed_nodes, num_slave_nodes, names_bytes;
+} cpi_inp_counts;
+int cpi_inp_read(const char *path, int32_t nodes_per_element,
+                 cpi_inp_counts *counts, double *xloc, int32_t *element_nodes,
+                 int32_t *node_set_starts, int32_t *node_set_indices,
+                 int32_t *fixed_nodes, int32_t *slave_nodes, char *names);
+
+/* Writes an .inp the reader above (and the reference's) accepts: *Node,
+   *Element (C3D4/C3D10), and an Nset=slave holding the given nodes. */
+int cpi_inp_write(const char *path, const cpi_mesh_desc *mesh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPPIMPACT_HIP_H */

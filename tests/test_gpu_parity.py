@@ -1,0 +1,184 @@
+"""GPU parity: the HIP step through the C ABI against the CPU oracle on the
+same seeded meshes (tolerances written per test)."""
+import os
+
+import numpy as np
+import pytest
+
+import cppimpact_amd as ci
+from cppimpact_amd import _lib as L
+from oracle import pyoracle as O
+
+from common import E, NU, RHO, cube, rel, run_gpu, run_oracle, wall_k
+
+pytestmark = pytest.mark.gpu
+
+SCATTERS = [L.SCATTER_BLOCK, L.SCATTER_COLOR, L.SCATTER_ATOMIC]
+
+
+def _disp_err(g, o, x0):
+    return rel(g["x"] - x0, o["x"] - x0)
+
+
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_compat_tet4_matches_oracle(scatter):
+    # the reference's step (5-point rule, row-sum mass, K_e du) with the wall
+    # engaged from the first steps
+    x, conn = cube(6, gap=1e-6)
+    k = wall_k(0.25, 6)
+    dt, n = 1e-7, 40
+    g = run_gpu(x, conn, (0, 0, -10.0), dt, n, k, scatter=scatter)
+    o = run_oracle(x, conn, (0, 0, -10.0), dt, n, O.COMPAT, k)
+    assert rel(g["m"], o["m"]) < 1e-13
+    assert _disp_err(g, o, x) < 1e-11
+    assert rel(g["v"], o["v"]) < 1e-11
+    assert rel(g["a"], o["a"]) < 1e-9
+    assert rel(g["v_i"], o["v_i"]) < 1e-11
+    e_o = o["energy"][-1]
+    assert abs(g["energy"][L.E_CONTACTF] - e_o[O.E_CONTACTF]) <= 1e-9 * abs(e_o[O.E_CONTACTF])
+    assert abs(g["energy"][L.E_KINETIC] - e_o[O.E_KINETIC]) <= 1e-11 * e_o[O.E_KINETIC]
+    assert abs(g["energy"][L.E_INTERNAL] - e_o[O.E_INTERNAL]) <= 1e-9 * abs(e_o[O.E_INTERNAL])
+    assert g["step"] == n
+    g["dyn"].close()
+
+
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_ep_tet4_f64_matches_oracle(scatter):
+    # 30 m/s impact: the contact face yields (rho c v ~ 500 MPa > 276 MPa)
+    x, conn = cube(6, gap=1e-6)
+    k = wall_k(0.25, 6)
+    dt, n = 2e-7, 80
+    g = run_gpu(x, conn, (0, 0, -30.0), dt, n, k, physics=L.PHYSICS_EP, scatter=scatter)
+    o = run_oracle(x, conn, (0, 0, -30.0), dt, n, O.EP, k)
+    sig, ep = g["dyn"].element_state()
+    assert o["epbar"].max() > 0, "the case must yield"
+    assert rel(g["m"], o["m"]) < 1e-13
+    assert _disp_err(g, o, x) < 1e-9
+    assert rel(g["v"], o["v"]) < 1e-9
+    assert rel(sig, o["sig"]) < 1e-8
+    assert rel(ep, o["epbar"]) < 1e-7
+    eo = o["energy"][-1]
+    for k_ in (O.E_KINETIC, O.E_INTERNAL, O.E_PLASTIC, O.E_PENALTY, O.E_GRAVITY):
+        assert abs(g["energy"][k_] - eo[k_]) <= 1e-8 * max(abs(eo[k_]), 1e-3 * eo[O.E_KINETIC])
+    g["dyn"].close()
+
+
+@pytest.mark.parametrize("physics,rule,nsteps", [
+    (L.PHYSICS_COMPAT, L.QUAD_REF5, 4),
+    (L.PHYSICS_EP, L.QUAD_EXACT4, 60),
+    (L.PHYSICS_EP, L.QUAD_REF4, 30),
+])
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_tet10_matches_oracle(physics, rule, nsteps, scatter):
+    x, conn = cube(3, quadratic=True, gap=1e-6)
+    k = wall_k(0.25, 6)
+    # compat tet10 carries the reference's negative corner masses: keep dt
+    # tiny and the run short
+    dt = 2e-9 if physics == L.PHYSICS_COMPAT else 5e-8
+    v0 = (0, 0, -30.0)
+    g = run_gpu(x, conn, v0, dt, nsteps, k, physics=physics, quadrature=rule, scatter=scatter)
+    o = run_oracle(x, conn, v0, dt, nsteps, O.COMPAT if physics == 0 else O.EP, k,
+                   quad_rule=rule)
+    assert rel(g["m"], o["m"]) < 1e-12
+    assert _disp_err(g, o, x) < 1e-9
+    assert rel(g["v"], o["v"]) < 1e-9
+    if physics == L.PHYSICS_EP:
+        sig, _ = g["dyn"].element_state()
+        assert rel(sig, o["sig"]) < 1e-8
+    g["dyn"].close()
+
+
+def test_f32_elements_within_1e3_of_fp64_oracle():
+    x, conn = cube(8, gap=1e-6, jitter=0.1, seed=3)
+    k = wall_k(0.25, 8)
+    dt, n = 2e-7, 150
+    g = run_gpu(x, conn, (0, 0, -30.0), dt, n, k, physics=L.PHYSICS_EP, precision=L.PREC_F32)
+    o = run_oracle(x, conn, (0, 0, -30.0), dt, n, O.EP, k)
+    d = _disp_err(g, o, x)
+    print(f"FP32 element kernels vs FP64 oracle after {n} steps: disp {d:.2e} "
+          f"vel {rel(g['v'], o['v']):.2e}")
+    assert d < 1e-3
+    assert rel(g["v"], o["v"]) < 1e-2
+    g["dyn"].close()
+
+
+def test_block_scatter_is_bitwise_reproducible_and_graph_equals_eager():
+    x, conn = cube(8, gap=1e-6, jitter=0.1, seed=1)
+    k = wall_k(0.25, 8)
+    runs = []
+    for graph in (1, 1, 0):
+        g = run_gpu(x, conn, (0, 0, -30.0), 2e-7, 50, k, physics=L.PHYSICS_EP, use_graph=graph)
+        runs.append(g)
+    for other in runs[1:]:
+        for key in ("x", "v", "a"):
+            assert np.array_equal(runs[0][key], other[key]), key
+    for g in runs:
+        g["dyn"].close()
+
+
+def test_coloring_bit_exact_with_oracle():
+    x, conn = cube(10, jitter=0.1, seed=7)
+    mesh = ci.Mesh.from_arrays(x, conn)
+    dyn = ci.Dynamics(mesh, ci.Elastoplastic(E, RHO, NU, 1.0, 700e6, 276e6), None,
+                      scatter=L.SCATTER_COLOR)
+    dyn.init(1e-7)
+    col, n = dyn.coloring()
+    ref, nref = O.greedy_coloring(conn, x.shape[0])
+    assert n == nref and np.array_equal(col, ref)
+    dyn.close()
+
+
+def test_stress_strain_output_matches_reference_update():
+    x, conn = cube(5, gap=1e-6)
+    k = wall_k(0.25, 5)
+    dt, n = 1e-7, 20
+    g = run_gpu(x, conn, (0, 0, -10.0), dt, n, k)
+    st, ss = g["dyn"].stress_strain()
+    s = O.Solver(x, conn, (0, 0, -10.0), E=E, nu=NU, rho=RHO, physics=O.COMPAT,
+                 wall=dict(location=0.0, stiffness=k), dt=dt)
+    o = s.solve(n, with_strain=True)
+    assert rel(st, o["strain"]) < 1e-8
+    assert rel(ss, o["stress"]) < 1e-8
+    g["dyn"].close()
+
+
+def test_debug_strain_known_answers_on_gpu(golden):
+    import json
+    vals = json.load(open(os.path.join(golden, "fea_debug_values.json")))["cases"]
+    for case in vals:
+        npe = 4 if case["basis"] == "linear" else 10
+        mesh = ci.Mesh(npe)
+        assert mesh.load_mesh(os.path.join(golden, case["mesh"])) == 0
+        dyn = ci.Dynamics(mesh, ci.Elastoplastic(E, RHO, NU, 0.0, 10.0, 1.9 * 3 ** 0.5), None)
+        W, V = dyn.debug_strain(0.01, case["def"])
+        assert abs(W - case["W"]) <= 1e-9 * case["W"], (case, W)
+        assert abs(V - case["V"]) <= 1e-6 * case["V"], (case, V)
+        dyn.close()
+
+
+def test_reference_fuselage_mesh_compat(golden):
+    # the reference driver's set-up (test.cu:57-95) on the 2647-element
+    # fuselage: wall at z = 0.15 - 5e-5, k = 10 E, v0 = (0, 0, -10), dt = 1e-7
+    mesh = ci.Mesh(4)
+    assert mesh.load_mesh(os.path.join(golden, "fuselage_2647.inp")) == 0
+    x, conn = mesh.xloc, mesh.element_nodes
+    dt, n = 1e-7, 12
+    g = run_gpu(x, conn, (0, 0, -10.0), dt, n, 10 * E, wall_location=0.15 - 0.00005)
+    o = run_oracle(x, conn, (0, 0, -10.0), dt, n, O.COMPAT, 10 * E,
+                   wall_location=0.15 - 0.00005)
+    assert _disp_err(g, o, x) < 1e-10
+    assert rel(g["v"], o["v"]) < 1e-10
+    g["dyn"].close()
+
+
+def test_solve_loop_count_matches_reference_time_loop():
+    # Dynamics::solve runs while (time <= time_end) with time += dt
+    x, conn = cube(3)
+    mesh = ci.Mesh.from_arrays(x, conn)
+    dyn = ci.Dynamics(mesh, ci.Elastoplastic(E, RHO, NU, 0.0, 10.0, 1.0), None)
+    dyn.vel[:] = (0.0, 0.0, -10.0)
+    dyn.solve(1e-7, 10 * 1e-7, 2**31 - 1)
+    assert dyn.info().step == ci.loop_iterations(1e-7, 1e-6)
+    o = run_oracle(x, conn, (0, 0, -10.0), 1e-7, dyn.info().step, O.COMPAT)
+    assert rel(dyn.global_xloc - x, o["x"] - x) < 1e-11
+    dyn.close()
