@@ -151,6 +151,10 @@ def main():
 
     from cppimpact_amd import _lib as L
 
+    import ctypes
+    gbs = ctypes.c_double(0.0)
+    L.check(L.lib().cpi_bandwidth_probe(0, 1 << 30, 20, ctypes.byref(gbs)))
+    print(f"bench: stream copy {gbs.value:.0f} GB/s", file=sys.stderr, flush=True)
     x, conn = make_mesh(args.n)
     print(f"bench: mesh {conn.shape[0]} tet4, {x.shape[0]} nodes", file=sys.stderr, flush=True)
     # one dt for both runs: half the stable step of the FP64 context
@@ -199,6 +203,7 @@ def main():
         "speedup_vs_fp64": round(mixed["value"] / plain["value"], 4),
         "disp_rel_err_vs_fp64": derr,
         "gpu_event_ms_per_step": round(mixed["gpu_ms_per_step"], 5),
+        "hbm_stream_copy_gbs": round(gbs.value, 1),
         "node_kernel_ms": round(mixed["node_ms"], 5),
         "blocks": mixed["blocks"], "boundary_nodes": mixed["boundary_nodes"],
         "epbar_max": mixed["epbar_max"],

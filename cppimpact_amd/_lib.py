@@ -112,6 +112,7 @@ SIGNATURES = {
     "cpi_profile_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(KernelTime), C.c_int32,
                                       C.POINTER(C.c_int32)]),
     "cpi_debug_strain": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _dp, _dp]),
+    "cpi_bandwidth_probe": (C.c_int, [C.c_int32, C.c_int64, C.c_int32, _dp]),
     "cpi_greedy_coloring": (C.c_int, [C.c_int32, C.c_int32, _ip, C.c_int32, _ip]),
     "cpi_mesh_box": (C.c_int, [C.POINTER(BoxSpec), _ip, _ip, _dp, _ip]),
     "cpi_mesh_cylinder": (C.c_int, [C.POINTER(CylSpec), _ip, _ip, _dp, _ip]),

@@ -39,7 +39,6 @@ struct cpi_ctx {
   std::vector<int32_t> color_off;           // colour scatter launch ranges (device order)
   BlockPlan plan;
   std::vector<void *> allocs;
-  double *d_X0[3] = {nullptr, nullptr, nullptr};
   double *d_tmp = nullptr;
   size_t tmp_len = 0;
   Params P{};
@@ -106,25 +105,27 @@ double rule_weight_sum(int rule) {
   }
 }
 
-// user AoS [N][3] -> device SoA (device node order)
-int put_nodes(cpi_ctx *c, double *const d[3], const double *aos) {
-  std::vector<double> tmp(c->nn);
-  for (int k = 0; k < 3; k++) {
-    for (int i = 0; i < c->nn; i++) tmp[i] = aos[3L * c->nperm[i] + k];
-    HIPCHK(c, hipMemcpyAsync(d[k], tmp.data(), sizeof(double) * c->nn, hipMemcpyHostToDevice,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+// user AoS [N][3] -> device 32-byte records (device node order), w = 0
+int put_nodes(cpi_ctx *c, Node4 *d, const double *aos) {
+  std::vector<Node4> tmp(c->nn);
+  for (int i = 0; i < c->nn; i++) {
+    const double *s = aos + 3L * c->nperm[i];
+    tmp[i] = Node4{s[0], s[1], s[2], 0.0};
   }
+  HIPCHK(c, hipMemcpyAsync(d, tmp.data(), sizeof(Node4) * c->nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return CPI_OK;
 }
 
-int get_nodes(cpi_ctx *c, double *const d[3], double *aos) {
-  std::vector<double> tmp(c->nn);
-  for (int k = 0; k < 3; k++) {
-    HIPCHK(c, hipMemcpyAsync(tmp.data(), d[k], sizeof(double) * c->nn, hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int i = 0; i < c->nn; i++) aos[3L * c->nperm[i] + k] = tmp[i];
+int get_nodes(cpi_ctx *c, const Node4 *d, double *aos) {
+  std::vector<Node4> tmp(c->nn);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), d, sizeof(Node4) * c->nn, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->nn; i++) {
+    double *o = aos + 3L * c->nperm[i];
+    o[0] = tmp[i].x;
+    o[1] = tmp[i].y;
+    o[2] = tmp[i].z;
   }
   return CPI_OK;
 }
@@ -197,35 +198,40 @@ int compute_mass(cpi_ctx *c) {
   const double rho = c->mat.rho;
   hipError_t e = hipSuccess;
   if (c->scatter == CPI_SCATTER_BLOCK) {
-    e = launch_mass(p, c->rule, 0, c->mass_kind, rho, (double *)p.m, c->nblocks, c->stream);
+    e = launch_mass(p, c->rule, 0, c->mass_kind, rho, c->nblocks, c->stream);
   } else if (c->scatter == CPI_SCATTER_ATOMIC) {
     p.elem_base = 0;
     p.elem_count = c->ne;
-    e = launch_mass(p, c->rule, 2, c->mass_kind, rho, (double *)p.m, 0, c->stream);
+    e = launch_mass(p, c->rule, 2, c->mass_kind, rho, 0, c->stream);
   } else {
     for (int k = 0; k < c->ncol && e == hipSuccess; k++) {
       p.elem_base = c->color_off[k];
       p.elem_count = c->color_off[k + 1] - c->color_off[k];
-      e = launch_mass(p, c->rule, 1, c->mass_kind, rho, (double *)p.m, 0, c->stream);
+      e = launch_mass(p, c->rule, 1, c->mass_kind, rho, 0, c->stream);
     }
   }
+  if (e == hipSuccess) e = launch_pack_mass(p, c->stream);
   if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("mass kernel: ") + hipGetErrorString(e));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mass_ready = true;
   return CPI_OK;
 }
 
+// x = X0 (mass kept in xm.w when assembled), v = v0, forces/aux/state zeroed
 int reset_state(cpi_ctx *c) {
-  int rc = put_nodes(c, c->P.x, c->X0u.data());
-  if (!rc) rc = put_nodes(c, c->P.v, c->V0u.data());
+  int rc = put_nodes(c, c->P.xm, c->X0u.data());
+  if (!rc) rc = put_nodes(c, c->P.vv, c->V0u.data());
   if (rc) return rc;
-  for (int k = 0; k < 3; k++) {
-    HIPCHK(c, hipMemsetAsync(c->P.f[k], 0, sizeof(double) * c->nn, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->P.acc[k], 0, sizeof(double) * c->nn, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->P.du[k], 0, sizeof(double) * c->nn, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->P.xprev[k], c->P.x[k], sizeof(double) * c->nn,
-                             hipMemcpyDeviceToDevice, c->stream));
+  if (c->mass_ready) {
+    hipError_t e = launch_pack_mass(c->P, c->stream);
+    if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
   }
+  for (int k = 0; k < 3; k++)
+    HIPCHK(c, hipMemsetAsync(c->P.f[k], 0, sizeof(double) * c->nn, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.acc, 0, sizeof(Node4) * c->nn, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->P.du, 0, sizeof(Node4) * c->nn, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->P.xprev, c->P.xm, sizeof(Node4) * c->nn, hipMemcpyDeviceToDevice,
+                           c->stream));
   const size_t rs = c->prec == CPI_PREC_F32 ? 4 : 8;
   if (c->phys != CPI_PHYSICS_COMPAT) {
     HIPCHK(c, hipMemsetAsync(c->P.sig, 0, rs * 6 * c->nq * (size_t)c->ne, c->stream));
@@ -381,18 +387,24 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   // device orders
   if (c->scatter == CPI_SCATTER_BLOCK) {
-    int me = opt->block_elements > 0 ? opt->block_elements : (npe == 4 ? 512 : 128);
-    // LDS budget: two workgroups per 160 KiB CU, never above the device's
-    // per-workgroup limit; node records take 64 bytes each
+    const int nt = elem_block_threads(npe);
+    int me = opt->block_elements > 0 ? opt->block_elements : nt;
+    // the tet4 kernel gives each thread one element and at most one node
+    if (npe == 4 && me > nt)
+      return fail(c, CPI_ERR_INVALID, "block_elements above " + std::to_string(nt) + " for tet4");
+    // LDS budget per workgroup: three (FP64) or four (FP32) tet4 workgroups,
+    // two tet10 ones per 160 KiB CU, never above the device's limit; node
+    // records take 64 bytes each
     const size_t rsz = c->prec == CPI_PREC_F32 ? 4 : 8;
-    const long budget = std::min<long>((long)prop.sharedMemPerBlock, 81920);
-    const long forces = (long)rsz * 3 * npe * me;
-    const long red = 8L * EP_N * (elem_block_threads(npe) / 64);
-    long mnl = (budget - forces - red - 64) / 64;
+    const long share = npe == 4 ? (c->prec == CPI_PREC_F32 ? 40960 : 54272) : 81920;
+    const long budget = std::min<long>((long)prop.sharedMemPerBlock, share);
+    const long fixed = (long)elem_block_smem(npe, c->prec, 0, me);
+    long mnl = (budget - fixed - 64) / 64;
     if (mnl < 4L * npe)
       return fail(c, CPI_ERR_INVALID, "block_elements " + std::to_string(me) +
                                           " does not fit the LDS budget");
-    int mn = (int)std::min<long>(mnl, 1024);
+    int mn = (int)std::min<long>(mnl, npe == 4 ? nt : 1024);
+    (void)rsz;
     std::string e;
     if (!build_block_plan(c->ne, npe, c->conn_u.data(), c->nn, c->X0u.data(), me, mn, c->plan, e))
       return fail(c, CPI_ERR_INVALID, "partition: " + e);
@@ -436,14 +448,14 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   int rc = CPI_OK;
   const int nn = c->nn, ne = c->ne;
   for (int kk = 0; kk < 3 && !rc; kk++) {
-    rc = dalloc(c, &P.x[kk], nn);
-    if (!rc) rc = dalloc(c, &P.v[kk], nn);
-    if (!rc) rc = dalloc(c, &P.f[kk], nn);
-    if (!rc) rc = dalloc(c, &P.acc[kk], nn);
-    if (!rc) rc = dalloc(c, &P.du[kk], nn);
-    if (!rc) rc = dalloc(c, &P.xprev[kk], nn);
-    if (!rc) rc = dalloc(c, &c->d_X0[kk], nn);
+    rc = dalloc(c, &P.f[kk], nn);
   }
+  if (!rc) rc = dalloc(c, &P.xm, nn);
+  if (!rc) rc = dalloc(c, &P.vv, nn);
+  if (!rc) rc = dalloc(c, &P.acc, nn);
+  if (!rc) rc = dalloc(c, &P.du, nn);
+  if (!rc) rc = dalloc(c, &P.xprev, nn);
+  if (!rc) rc = dalloc(c, &P.X0, nn);
   double *dm = nullptr;
   if (!rc) rc = dalloc(c, &dm, nn);
   P.m = dm;
@@ -481,20 +493,20 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     BlockPlan &B = c->plan;
     int32_t *meta, *bn, *bs, *sp;
     uint16_t *cl, *cp, *ci;
-    double *part;
+    Node4 *part;
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||
         (rc = dalloc(c, &cp, B.csr_ptr.size())) || (rc = upload(c, cp, B.csr_ptr)) ||
-        (rc = dalloc(c, &ci, B.csr_idx.size())) || (rc = upload(c, ci, B.csr_idx)) ||
+        (rc = dalloc(c, &ci, B.csr_pos.size())) || (rc = upload(c, ci, B.csr_pos)) ||
         (rc = dalloc(c, &bn, B.bnd_nodes.size())) || (rc = upload(c, bn, B.bnd_nodes)) ||
         (rc = dalloc(c, &bs, B.bnd_slot.size())) || (rc = upload(c, bs, B.bnd_slot)) ||
         (rc = dalloc(c, &sp, B.slot_ptr.size())) || (rc = upload(c, sp, B.slot_ptr)) ||
-        (rc = dalloc(c, &part, 3 * (size_t)std::max(B.n_slots, 1))))
+        (rc = dalloc(c, &part, (size_t)std::max(B.n_slots, 1))))
       return rc;
     P.meta = meta;
     P.conn_local = cl;
     P.csr_ptr = cp;
-    P.csr_idx = ci;
+    P.csr_pos = ci;
     P.bnd_nodes = bn;
     P.bnd_slot = bs;
     P.slot_ptr = sp;
@@ -511,30 +523,33 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     if (pe != hipSuccess)
       return fail(c, CPI_ERR_HIP, std::string("LDS opt-in for the element kernel: ") +
                                       hipGetErrorString(pe));
-    c->n_elem_rows = c->nblocks;
-    c->n_node_rows = std::max(1, (B.n_boundary + 255) / 256);
+    // one energy row per wave
+    c->n_elem_rows = c->nblocks * (elem_block_threads(npe) / 64);
+    c->n_node_rows = 4 * std::max(1, (B.n_boundary + 255) / 256);
   } else {
-    const int nt = npe == 4 ? 256 : 64;
+    const int nt = elem_direct_threads(npe);
     c->color_rows.clear();
     int rows = 0;
     if (c->scatter == CPI_SCATTER_COLOR) {
       for (int kk = 0; kk < c->ncol; kk++) {
         c->color_rows.push_back(rows);
-        rows += (c->color_off[kk + 1] - c->color_off[kk] + nt - 1) / nt;
+        rows += (nt / 64) * ((c->color_off[kk + 1] - c->color_off[kk] + nt - 1) / nt);
       }
     } else {
       c->color_rows.push_back(0);
-      rows = (ne + nt - 1) / nt;
+      rows = (nt / 64) * ((ne + nt - 1) / nt);
     }
     c->n_elem_rows = rows;
-    c->n_node_rows = (nn + 255) / 256;
+    c->n_node_rows = 4 * ((nn + 255) / 256);
   }
-  c->n_rows = std::max(c->n_elem_rows + c->n_node_rows, (nn + 255) / 256);
-  double *ep = nullptr, *hist = nullptr;
+  c->n_rows = std::max(c->n_elem_rows + c->n_node_rows, 4 * ((nn + 255) / 256));
+  double *ep = nullptr, *hist = nullptr, *ered = nullptr;
   Ctl *ctl = nullptr;
   if ((rc = dalloc(c, &ep, (size_t)EP_N * c->n_rows)) || (rc = dalloc(c, &ctl, 1)) ||
-      (rc = dalloc(c, &hist, 8 * 4096)))
+      (rc = dalloc(c, &hist, 8 * 4096)) ||
+      (rc = dalloc(c, &ered, (size_t)EP_N * energy_reduce_blocks())))
     return rc;
+  P.ered = ered;
   P.epart = ep;
   P.n_epart = c->n_rows;
   P.ctl = ctl;
@@ -547,7 +562,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     P.mlp_stats = c->mlp.d_stats;
   }
   // reference positions on the device (stress output)
-  if ((rc = put_nodes(c, c->d_X0, c->X0u.data()))) return rc;
+  if ((rc = put_nodes(c, c->P.X0, c->X0u.data()))) return rc;
   return reset_state(c);
 }
 
@@ -600,7 +615,7 @@ extern "C" int cpi_set_state(cpi_ctx *c, const double *x0, const double *v0) {
   }
   c->inited = false;
   drop_graphs(c);
-  int rc = put_nodes(c, c->d_X0, c->X0u.data());
+  int rc = put_nodes(c, c->P.X0, c->X0u.data());
   if (rc) return rc;
   return reset_state(c);
 }
@@ -674,11 +689,11 @@ extern "C" int cpi_get_state(cpi_ctx *c, double *x, double *v, double *a, double
   if (!c) return CPI_ERR_INVALID;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int rc = CPI_OK;
-  if (x && (rc = get_nodes(c, c->P.x, x))) return rc;
+  if (x && (rc = get_nodes(c, c->P.xm, x))) return rc;
   std::vector<double> hv, ha;
   if (v || vi) {
     hv.resize(3L * c->nn);
-    if ((rc = get_nodes(c, c->P.v, hv.data()))) return rc;
+    if ((rc = get_nodes(c, c->P.vv, hv.data()))) return rc;
     if (v) std::memcpy(v, hv.data(), sizeof(double) * hv.size());
   }
   if (a || vi) {
@@ -738,8 +753,7 @@ extern "C" int cpi_get_stress_strain(cpi_ctx *c, double *strain, double *stress)
   if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_get_stress_strain before cpi_init");
   int rc = ensure_tmp(c, 12 * (size_t)c->ne);
   if (rc) return rc;
-  const double *X0[3] = {c->d_X0[0], c->d_X0[1], c->d_X0[2]};
-  hipError_t e = launch_stress_elem(c->P, X0, c->d_tmp, c->stream);
+  hipError_t e = launch_stress_elem(c->P, c->d_tmp, c->stream);
   if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
   std::vector<double> h(12 * (size_t)c->ne), byuser(12 * (size_t)c->ne);
   HIPCHK(c, hipMemcpyAsync(h.data(), c->d_tmp, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -904,7 +918,6 @@ extern "C" int cpi_debug_strain(cpi_ctx *c, double alpha, int32_t dcase, double 
   int rc = ensure_tmp(c, 2 * (size_t)nb);
   if (rc) return rc;
   Params p = c->P;
-  for (int k = 0; k < 3; k++) p.x[k] = c->d_X0[k];
   hipError_t e = launch_debug_strain(p, c->rule, alpha, dcase, c->mat.nu, c->d_tmp, c->stream);
   if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
   std::vector<double> h(2 * (size_t)nb);
@@ -918,4 +931,49 @@ extern "C" int cpi_debug_strain(cpi_ctx *c, double alpha, int32_t dcase, double 
   *W = w;
   *V = v;
   return CPI_OK;
+}
+
+extern "C" int cpi_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs) {
+  if (!gbs || bytes < 1024 || reps < 1) {
+    set_global_error("cpi_bandwidth_probe: invalid arguments");
+    return CPI_ERR_INVALID;
+  }
+  cpi_ctx tmp;  // only for error reporting through HIPCHK
+  cpi_ctx *c = &tmp;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    set_global_error("cpi_bandwidth_probe: no HIP device " + std::to_string(device));
+    return CPI_ERR_NO_DEVICE;
+  }
+  const long n4 = (long)(bytes / 32);
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = CPI_OK;
+  auto run = [&]() -> int {
+    HIPCHK(c, hipSetDevice(device));
+    HIPCHK(c, hipMalloc(&a, n4 * 32));
+    HIPCHK(c, hipMalloc(&b, n4 * 32));
+    HIPCHK(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(c, hipMemsetAsync(a, 0, n4 * 32, s));
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, launch_stream_copy(a, b, n4, s));  // warm-up
+    HIPCHK(c, hipEventRecord(e0, s));
+    for (int i = 0; i < reps; i++) HIPCHK(c, launch_stream_copy(a, b, n4, s));
+    HIPCHK(c, hipEventRecord(e1, s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *gbs = 2.0 * 32.0 * (double)n4 * reps / (ms * 1e-3) / 1e9;
+    return CPI_OK;
+  };
+  rc = run();
+  if (rc) set_global_error(c->err);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }

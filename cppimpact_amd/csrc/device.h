@@ -35,6 +35,10 @@ struct Ctl {
 
 struct MlpParams;  // surrogate, see mlp.h
 
+struct alignas(32) Node4 {
+  double x, y, z, w;
+};
+
 struct Params {
   int nn, ne, npe, nq;
   // material (D per tetrahedral.h:486-512: c11 = (1-nu) k, c12 = nu k,
@@ -49,29 +53,32 @@ struct Params {
   double qw[kMaxQ];
   double qpt[kMaxQ][3];
   double wsum;
-  // nodal state, device node order, structure of arrays
-  double *x[3];
-  double *v[3];
-  const double *m;
+  // nodal state, device node order: one 32-byte record per node per field,
+  // so a gathered node costs one cache line per field
+  Node4 *xm;             // (x, y, z, m)
+  Node4 *vv;             // (vx, vy, vz, 0) = v_{n+1/2}
+  double *m;             // lumped mass as assembled (copied into xm.w)
   double *f[3];          // assembled force (colour / atomic scatter)
-  double *acc[3];        // a_n            written when store_aux
-  double *du[3];         // du of the step written when store_aux
-  double *xprev[3];      // x_n before the step, written when store_aux
+  Node4 *acc;            // a_n                   written when store_aux
+  Node4 *du;             // du of the step        written when store_aux
+  Node4 *xprev;          // x_n before the step   written when store_aux
+  Node4 *X0;             // reference positions
   const int32_t *conn;   // [npe][ne] device node ids (colour / atomic)
   void *sig;             // [nq][6][ne] stress state (ep)
   void *epbar;           // [nq][ne]
   // workgroup plan (LDS-assembled scatter)
   const int32_t *meta;
   const uint16_t *conn_local;
-  const uint16_t *csr_ptr;
-  const uint16_t *csr_idx;
+  const uint16_t *csr_ptr;   // per block: local node -> first force position
+  const uint16_t *csr_pos;   // [E][npe]: corner -> force position in its block
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;
   const int32_t *slot_ptr;
-  double *partial;       // [slots][3]
+  Node4 *partial;        // one force partial per slot (x, y, z, -)
   int n_interior, n_boundary, max_loc, max_el;
   // energy partial rows and control block
-  double *epart;
+  double *epart;         // one row of EP_N doubles per wave of the step kernels
+  double *ered;          // first-stage sums of the rows
   int epart_base;        // first row of this launch
   int n_epart;
   Ctl *ctl;

@@ -1,5 +1,10 @@
 // Element physics on the device: tet4 and tet10 internal force, stress
 // update, lumped mass and the stable-dt estimate.  Included by kernels.hip.
+//
+// The element cores read nodes through a getter get(k, x[3], v[3]) and hand
+// corner forces to a putter put(k, f[3]), so the same physics serves the
+// LDS-staged workgroup kernels and the one-element-per-thread kernels
+// without holding every node record in registers.
 #pragma once
 
 #include "device.h"
@@ -40,11 +45,6 @@ template <> struct Quad<RULE_CENTROID> {
   __device__ static constexpr double pt(int, int) { return 0.25; }
 };
 
-// number of stress-state points an element carries
-template <int NPE, int RULE> struct NState {
-  static constexpr int n = NPE == 4 ? 1 : Quad<RULE>::n;
-};
-
 template <typename R>
 __device__ __forceinline__ void load_state(const Params &p, int e, int q, R s[6], R &ep) {
   const R *S = (const R *)p.sig;
@@ -65,13 +65,21 @@ __device__ __forceinline__ void store_state(const Params &p, int e, int q, const
   P[(long)q * ne + e] = ep;
 }
 
+// State of one point loaded ahead of use (the tet4 workgroup kernel issues
+// it before its first barrier so the load overlaps the node staging).
+template <typename R> struct PreState {
+  R s[6];
+  R ep;
+};
+
 // Stress at one point from the strain increment de.  compat: sigma = D de
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
 // point's internal work and plastic dissipation (times wd) to ea.
-template <typename R, int PHYS>
+template <typename R, int PHYS, bool PRE = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
-                                             R wd, R s[6], EAcc &ea) {
+                                             R wd, R s[6], EAcc &ea,
+                                             const PreState<R> *pre = nullptr) {
   const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
   R ds[6];
   ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
@@ -90,9 +98,21 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     ea.wint += double(wd * w);
   } else {
     R so[6], ep;
-    load_state<R>(p, e, q, so, ep);
+    if constexpr (PRE) {
 #pragma unroll
-    for (int c = 0; c < 6; c++) s[c] = so[c] + ds[c];
+      for (int c = 0; c < 6; c++) so[c] = pre->s[c];
+      ep = pre->ep;
+    } else {
+      load_state<R>(p, e, q, so, ep);
+    }
+    // midpoint internal work 1/2 (sigma_n + sigma_{n+1}) : de, split so that
+    // sigma_n need not stay live through the return map
+    R w = R(0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      w += so[c] * de[c];
+      s[c] = so[c] + ds[c];
+    }
     R dg;
     if constexpr (PHYS == PHYS_EP_MLP) {
       dg = mlp_return<R>(p, de, so, ep, s);
@@ -101,9 +121,8 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
       dg = radial_return<R>(p, s, ep);
       if (p.samples && dg > R(0)) record_sample<R>(p, de, so, ep0, s, ep);
     }
-    R w = R(0);
 #pragma unroll
-    for (int c = 0; c < 6; c++) w += (so[c] + s[c]) * de[c];
+    for (int c = 0; c < 6; c++) w += s[c] * de[c];
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
@@ -111,7 +130,7 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
   }
 }
 
-// f_i = wd * sigma . dN_i
+// f_i += wd * sigma . dN_i
 template <typename R>
 __device__ __forceinline__ void add_bts(R f[3], const R dn[3], const R s[6], R wd) {
   f[0] += wd * (dn[0] * s[0] + dn[1] * s[3] + dn[2] * s[5]);
@@ -136,30 +155,45 @@ __device__ __forceinline__ double corner_dt(const R e1[3], const R e2[3], const 
   return double(fabs(vol6)) / sqrt(double(a)) / (c * factor);
 }
 
+// characteristic length = minimum corner altitude / factor; the bare
+// altitude overshoots the exact element limit 2/omega_max (stiffness against
+// lumped mass) by up to 1.91x for tet4, hence 2; 4.35 keeps tet10 below it
+constexpr double kTet4LengthFactor = 2.0;
 constexpr double kTet10LengthFactor = 4.35;
 
 // ---------------------------------------------------------------------------
 // tet4: B is constant, every quadrature point carries the same strain, so
 // the rule enters only through its weight sum (compat) and the element keeps
-// one stress point (ep).
+// one stress point (ep).  Corner forces reach put() as soon as known.
 // ---------------------------------------------------------------------------
-template <typename R, int PHYS>
-__device__ __forceinline__ void tet4_force(const Params &p, const double (&xs)[4][3],
-                                           const double (&vs)[4][3], double dt, int e,
-                                           R (&f)[4][3], EAcc &ea) {
+template <typename R, int PHYS, bool PRE, class Get, class Put>
+__device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
+                                          const Put &put, EAcc &ea,
+                                          const PreState<R> *pre = nullptr) {
   R J[9];
+  R u[3][3];
+  {
+    double x0[3], v0[3];
+    get(0, x0, v0);
 #pragma unroll
-  for (int a = 0; a < 3; a++)
+    for (int k = 1; k < 4; k++) {
+      double xk[3], vk[3];
+      get(k, xk, vk);
 #pragma unroll
-    for (int q = 0; q < 3; q++) J[a * 3 + q] = R(xs[q + 1][a] - xs[0][a]);
+      for (int a = 0; a < 3; a++) {
+        J[a * 3 + (k - 1)] = R(xk[a] - x0[a]);
+        u[k - 1][a] = R(dt * vk[a] - dt * v0[a]);
+      }
+    }
+  }
   R Ji[9];
   R detJ = inv3<R>(J, Ji);
   if (!(detJ > R(0))) ea.bad += 1.0;
-  R u[3][3];
-#pragma unroll
-  for (int k = 0; k < 3; k++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) u[k][c] = R(dt * vs[k + 1][c] - dt * vs[0][c]);
+  if constexpr (PHYS != PHYS_COMPAT) {
+    R e1[3] = {J[0], J[3], J[6]}, e2[3] = {J[1], J[4], J[7]}, e3[3] = {J[2], J[5], J[8]};
+    double d = corner_dt<R>(e1, e2, e3, detJ, p.wave_speed, kTet4LengthFactor);
+    ea.mindt = fmin(ea.mindt, d);
+  }
   // L[i][j] = d du_i / d x_j
   R L[3][3];
 #pragma unroll
@@ -171,41 +205,36 @@ __device__ __forceinline__ void tet4_force(const Params &p, const double (&xs)[4
              L[0][2] + L[2][0]};
   R wd = R(p.wsum) * detJ;
   R s[6];
-  point_stress<R, PHYS>(p, e, 0, de, wd, s, ea);
-#pragma unroll
-  for (int c = 0; c < 3; c++) f[0][c] = R(0);
+  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre);
+  R f0[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int k = 1; k < 4; k++) {
     R dn[3] = {Ji[(k - 1) * 3 + 0], Ji[(k - 1) * 3 + 1], Ji[(k - 1) * 3 + 2]};
-    f[k][0] = f[k][1] = f[k][2] = R(0);
-    add_bts<R>(f[k], dn, s, wd);
+    R fk[3] = {R(0), R(0), R(0)};
+    add_bts<R>(fk, dn, s, wd);
 #pragma unroll
-    for (int c = 0; c < 3; c++) f[0][c] -= f[k][c];
+    for (int c = 0; c < 3; c++) f0[c] -= fk[c];
+    put(k, fk);
   }
-  if constexpr (PHYS != PHYS_COMPAT) {
-    R e1[3] = {J[0], J[3], J[6]}, e2[3] = {J[1], J[4], J[7]}, e3[3] = {J[2], J[5], J[8]};
-    double d = corner_dt<R>(e1, e2, e3, detJ, p.wave_speed, 1.0);
-    ea.mindt = fmin(ea.mindt, d);
-  }
+  put(0, f0);
 }
 
 // ---------------------------------------------------------------------------
-// tet10: loop over the rule's points; coordinates relative to corner 2 (the
-// origin of the reference's quadratic parametrisation)
+// tet10: loop over the rule's points with coordinates relative to corner 2
+// (the origin of the reference's quadratic parametrisation); nodes are
+// re-read through get() at every pass instead of being held in registers.
 // ---------------------------------------------------------------------------
-template <int RULE, typename R, int PHYS>
-__device__ __forceinline__ void tet10_force(const Params &p, const double (&xs)[10][3],
-                                            const double (&vs)[10][3], double dt, int e,
-                                            R (&f)[10][3], EAcc &ea) {
-  R xr[10][3], ur[10][3];
+template <int RULE, typename R, int PHYS, class Get, class Put>
+__device__ __forceinline__ void tet10_core(const Params &p, const Get &get, double dt, int e,
+                                           const Put &put, EAcc &ea) {
+  double o[3];
+  {
+    double ov[3];
+    get(2, o, ov);
+  }
+  R f[10][3];
 #pragma unroll
-  for (int i = 0; i < 10; i++)
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      xr[i][c] = R(xs[i][c] - xs[2][c]);
-      ur[i][c] = R(dt * vs[i][c]);
-      f[i][c] = R(0);
-    }
+  for (int i = 0; i < 10; i++) f[i][0] = f[i][1] = f[i][2] = R(0);
   using Q = Quad<RULE>;
 #pragma unroll
   for (int q = 0; q < Q::n; q++) {
@@ -213,28 +242,36 @@ __device__ __forceinline__ void tet10_force(const Params &p, const double (&xs)[
     tet10_grad(Q::pt(q, 0), Q::pt(q, 1), Q::pt(q, 2), Gd);
     R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
-    for (int i = 0; i < 10; i++)
+    for (int i = 0; i < 10; i++) {
+      double xi[3], vi[3];
+      get(i, xi, vi);
 #pragma unroll
-      for (int a = 0; a < 3; a++)
+      for (int a = 0; a < 3; a++) {
+        const R d = R(xi[a] - o[a]);
 #pragma unroll
-        for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * xr[i][a];
+        for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * d;
+      }
+    }
     R Ji[9];
     R detJ = inv3<R>(J, Ji);
     if (!(detJ > R(0))) ea.bad += 1.0;
     R de[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
     for (int i = 0; i < 10; i++) {
+      double xi[3], vi[3];
+      get(i, xi, vi);
+      R u[3] = {R(dt * vi[0]), R(dt * vi[1]), R(dt * vi[2])};
       R dn[3];
 #pragma unroll
       for (int c = 0; c < 3; c++)
         dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
                 Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
-      de[0] += dn[0] * ur[i][0];
-      de[1] += dn[1] * ur[i][1];
-      de[2] += dn[2] * ur[i][2];
-      de[3] += dn[1] * ur[i][0] + dn[0] * ur[i][1];
-      de[4] += dn[2] * ur[i][1] + dn[1] * ur[i][2];
-      de[5] += dn[2] * ur[i][0] + dn[0] * ur[i][2];
+      de[0] += dn[0] * u[0];
+      de[1] += dn[1] * u[1];
+      de[2] += dn[2] * u[2];
+      de[3] += dn[1] * u[0] + dn[0] * u[1];
+      de[4] += dn[2] * u[1] + dn[1] * u[2];
+      de[5] += dn[2] * u[0] + dn[0] * u[2];
     }
     R wd = R(Q::w(q)) * detJ;
     R s[6];
@@ -252,26 +289,31 @@ __device__ __forceinline__ void tet10_force(const Params &p, const double (&xs)[
   if constexpr (PHYS != PHYS_COMPAT) {
     // corners in the linear ordering: (2, 0, 1, 3) is an even permutation
     R e1[3], e2[3], e3[3];
+    double xa[3], va[3];
+    get(0, xa, va);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      e1[c] = xr[0][c];
-      e2[c] = xr[1][c];
-      e3[c] = xr[3][c];
-    }
+    for (int c = 0; c < 3; c++) e1[c] = R(xa[c] - o[c]);
+    get(1, xa, va);
+#pragma unroll
+    for (int c = 0; c < 3; c++) e2[c] = R(xa[c] - o[c]);
+    get(3, xa, va);
+#pragma unroll
+    for (int c = 0; c < 3; c++) e3[c] = R(xa[c] - o[c]);
     R v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
            e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
     ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, e3, v6, p.wave_speed, kTet10LengthFactor));
   }
+#pragma unroll
+  for (int i = 0; i < 10; i++) put(i, f[i]);
 }
 
-template <int NPE, int RULE, typename R, int PHYS>
-__device__ __forceinline__ void elem_force(const Params &p, const double (&xs)[NPE][3],
-                                           const double (&vs)[NPE][3], double dt, int e,
-                                           R (&f)[NPE][3], EAcc &ea) {
+template <int NPE, int RULE, typename R, int PHYS, class Get, class Put>
+__device__ __forceinline__ void elem_core(const Params &p, const Get &get, double dt, int e,
+                                          const Put &put, EAcc &ea) {
   if constexpr (NPE == 4)
-    tet4_force<R, PHYS>(p, xs, vs, dt, e, f, ea);
+    tet4_core<R, PHYS, false>(p, get, dt, e, put, ea);
   else
-    tet10_force<RULE, R, PHYS>(p, xs, vs, dt, e, f, ea);
+    tet10_core<RULE, R, PHYS>(p, get, dt, e, put, ea);
 }
 
 // ---------------------------------------------------------------------------
@@ -279,8 +321,8 @@ __device__ __forceinline__ void elem_force(const Params &p, const double (&xs)[N
 // points; HRZ gives rho V/4 (tet4), rho V/36 and 4 rho V/27 (tet10)
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE>
-__device__ __forceinline__ void elem_mass(const Params &p, const double (&xs)[NPE][3], int kind,
-                                          double rho, double (&me)[NPE]) {
+__device__ __forceinline__ void elem_mass(const double (&xs)[NPE][3], int kind, double rho,
+                                          double (&me)[NPE]) {
   double e1[3], e2[3], e3[3];
   const int o = NPE == 4 ? 0 : 2, i1 = NPE == 4 ? 1 : 0, i2 = NPE == 4 ? 2 : 1;
 #pragma unroll

@@ -233,7 +233,7 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
   P.meta.assign((size_t)nb * 8, 0);
   P.elem_perm.assign(ne, 0);
   P.conn_local.assign((size_t)ne * npe, 0);
-  P.csr_idx.assign((size_t)ne * npe, 0);
+  P.csr_pos.assign((size_t)ne * npe, 0);
   P.csr_ptr.clear();
   P.bnd_nodes.clear();
   P.bnd_slot.clear();
@@ -279,7 +279,7 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
     for (int t = 0; t < nel; t++)
       for (int j = 0; j < npe; j++) {
         int l = P.conn_local[(size_t)(es + t) * npe + j];
-        P.csr_idx[(size_t)es * npe + pos[l]++] = (uint16_t)(t * npe + j);
+        P.csr_pos[(size_t)(es + t) * npe + j] = (uint16_t)(pos[l]++);
       }
     P.max_local_nodes = std::max(P.max_local_nodes, nl);
     P.max_block_elems = std::max(P.max_block_elems, nel);

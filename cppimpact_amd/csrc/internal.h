@@ -40,8 +40,11 @@ struct BlockPlan {
   std::vector<uint16_t> conn_local;  // [E][npe] local node index
   std::vector<int32_t> bnd_nodes;    // per block boundary nodes (device id)
   std::vector<int32_t> bnd_slot;     // parallel to bnd_nodes: partial slot
+  // Corner forces of a block are stored grouped by local node: node l owns
+  // positions [csr_ptr[l], csr_ptr[l+1]) in element order, and corner j of
+  // element t writes to position csr_pos[t*npe + j].
   std::vector<uint16_t> csr_ptr;     // per block n_loc+1 offsets
-  std::vector<uint16_t> csr_idx;     // contribution = elem_local*npe + corner
+  std::vector<uint16_t> csr_pos;     // [E][npe] position of each corner force
   std::vector<int32_t> slot_ptr;     // per boundary node, n_boundary+1
 };
 

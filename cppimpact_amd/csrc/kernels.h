@@ -9,6 +9,7 @@ namespace cpi {
 
 size_t elem_block_smem(int npe, int prec, int max_loc, int max_el);
 int elem_block_threads(int npe);
+int elem_direct_threads(int npe);
 
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,
                              int nblocks, size_t smem, hipStream_t s);
@@ -19,12 +20,16 @@ hipError_t launch_node_bnd(const Params &p, bool aux, hipStream_t s);
 hipError_t launch_node_all(const Params &p, bool aux, hipStream_t s);
 hipError_t launch_node_init(const Params &p, hipStream_t s);
 hipError_t launch_energy_reduce(const Params &p, hipStream_t s);
-hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho, double *m,
+int energy_reduce_blocks();  // rows of p.ered (EP_N doubles each)
+// lumped mass into p.m (scatter: 0 block, 1 colour range, 2 atomic)
+hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho,
                        int nblocks, hipStream_t s);
+hipError_t launch_pack_mass(const Params &p, hipStream_t s);
 hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s);
-hipError_t launch_stress_elem(const Params &p, const double *const X0[3], double *out,
-                              hipStream_t s);
+hipError_t launch_stress_elem(const Params &p, double *out, hipStream_t s);
 hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcase, double nu,
                                double *out, hipStream_t s);
+// copies n4 32-byte records
+hipError_t launch_stream_copy(const void *src, void *dst, long n4, hipStream_t s);
 
 }  // namespace cpi

@@ -2,7 +2,8 @@
 //
 // One loop iteration of Dynamics::solve (dynamics.h:471-511) is, with the
 // LDS-assembled scatter (the default):
-//   k_elem_block  one workgroup per element block: stage the block's nodes
+//   k_elem_block4 / k_elem_block
+//                 one workgroup per element block: stage the block's nodes
 //                 in LDS, compute element forces (and the stress update),
 //                 sum them per node in a fixed order, finish the nodes owned
 //                 by the block (wall, gravity, leapfrog) and write one
@@ -13,7 +14,11 @@
 // colour (plain read-modify-write, no two elements of a colour share a
 // node); with the atomic scatter it is one launch using FP64 atomics; both
 // are followed by k_node_all.
+//
+// Nodal state lives in 32-byte records: xm = (x, y, z, m), vv = (v, 0).
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "device.h"
 #include "elem.h"
@@ -37,37 +42,38 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-// Deterministic block reduction of the energy accumulators; the result is
-// valid in thread 0.  red must hold 8 * (blockDim/64) doubles.
-__device__ __forceinline__ void block_reduce(EAcc &ea, double *red, double out[EP_N]) {
-  double v[EP_N] = {ea.wint, ea.wpl, ea.mindt, ea.bad, ea.ke, ea.pen, ea.fc, ea.wgrav};
-#pragma unroll
-  for (int k = 0; k < EP_N; k++) v[k] = (k == EP_MINDT) ? wave_min(v[k]) : wave_sum(v[k]);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  __syncthreads();
-  if (lane == 0)
-#pragma unroll
-    for (int k = 0; k < EP_N; k++) red[w * EP_N + k] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < EP_N; k++) out[k] = red[k];
-    for (int i = 1; i < nw; i++)
-      for (int k = 0; k < EP_N; k++)
-        out[k] = (k == EP_MINDT) ? fmin(out[k], red[i * EP_N + k]) : out[k] + red[i * EP_N + k];
+// Each wave owns one energy row: its lanes' accumulators are reduced with
+// cross-lane shuffles (fixed order, so deterministic) and lane 0 updates the
+// row; no workgroup barrier is involved.  Cumulative terms (internal and
+// plastic work, gravity work, inverted-element count) accumulate in the row
+// every step; the instantaneous ones (kinetic and penalty energy, contact
+// force, stable dt) are only reduced on the steps that end a batch, the only
+// ones whose rows are summed.  The row of a wave is base + its wave index in
+// the grid.
+template <bool ALL>
+__device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc &ea) {
+  const double wint = wave_sum(ea.wint), wpl = wave_sum(ea.wpl), bad = wave_sum(ea.bad),
+               wg = wave_sum(ea.wgrav);
+  double ke = 0.0, pen = 0.0, fc = 0.0, md = 0.0;
+  if (ALL) {
+    ke = wave_sum(ea.ke);
+    pen = wave_sum(ea.pen);
+    fc = wave_sum(ea.fc);
+    md = wave_min(ea.mindt);
   }
-}
-
-// cumulative terms accumulate in the row, instantaneous ones overwrite it
-__device__ __forceinline__ void write_row(const Params &p, int row, const double v[EP_N]) {
-  double *r = p.epart + (long)row * EP_N;
-  r[EP_WINT] += v[EP_WINT];
-  r[EP_WPL] += v[EP_WPL];
-  r[EP_BAD] += v[EP_BAD];
-  r[EP_WGRAV] += v[EP_WGRAV];
-  r[EP_MINDT] = v[EP_MINDT];
-  r[EP_KE] = v[EP_KE];
-  r[EP_PEN] = v[EP_PEN];
-  r[EP_FC] = v[EP_FC];
+  if ((threadIdx.x & 63) == 0) {
+    double *r = p.epart + (long)(base + (threadIdx.x >> 6)) * EP_N;
+    r[EP_WINT] += wint;
+    r[EP_WPL] += wpl;
+    r[EP_BAD] += bad;
+    r[EP_WGRAV] += wg;
+    if (ALL) {
+      r[EP_MINDT] = md;
+      r[EP_KE] = ke;
+      r[EP_PEN] = pen;
+      r[EP_FC] = fc;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -99,32 +105,43 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
       }
     }
   }
-  double ke = 0.0, wg = 0.0;
+  double ke = 0.0, wg = 0.0, xn[3], vn[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     a[c] += p.grav[c];
-    double vn = v[c] + dt * a[c];
-    double vi = vn - 0.5 * dt * a[c];
+    vn[c] = v[c] + dt * a[c];
+    double vi = vn[c] - 0.5 * dt * a[c];
     ke += vi * vi;
     wg += p.grav[c] * du[c];
-    p.x[c][g] = x[c] + du[c];
-    p.v[c][g] = vn;
-    if (AUX) {
-      p.acc[c][g] = a[c];
-      p.du[c][g] = du[c];
-      p.xprev[c][g] = x[c];
-    }
+    xn[c] = x[c] + du[c];
+  }
+  p.xm[g] = Node4{xn[0], xn[1], xn[2], m};
+  p.vv[g] = Node4{vn[0], vn[1], vn[2], 0.0};
+  if (AUX) {
+    p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
+    p.du[g] = Node4{du[0], du[1], du[2], 0.0};
+    p.xprev[g] = Node4{x[0], x[1], x[2], m};
   }
   ea.ke += 0.5 * m * ke;
   ea.wgrav += m * wg;
 }
 
-// ---------------------------------------------------------------------------
-// LDS-assembled element pass
-// ---------------------------------------------------------------------------
+// LDS node record: (x, y, z, m, vx, vy, vz, -)
+__device__ __forceinline__ void stage_node(double *d, const Node4 &xm, const Node4 &vv) {
+  d[0] = xm.x;
+  d[1] = xm.y;
+  d[2] = xm.z;
+  d[3] = xm.w;
+  d[4] = vv.x;
+  d[5] = vv.y;
+  d[6] = vv.z;
+}
 
-// LDS: node records [max_loc][8] doubles (x, v, -, -), then the element
-// corner forces [max_el*NPE][3] (R), then the reduction scratch.
+// ---------------------------------------------------------------------------
+// LDS-assembled element pass, generic (tet10): loops over elements and
+// nodes.  LDS: node records [max_loc][8] doubles, corner forces
+// [max_el*NPE][3] R, reduction scratch.
+// ---------------------------------------------------------------------------
 template <int NPE, int RULE, typename R, int PHYS, bool AUX, int NT>
 __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   extern __shared__ double smem[];
@@ -133,102 +150,161 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
             coff = mt[6], loff = mt[7];
   const int nloc = nint + nbnd;
+  (void)coff;
   double *lnode = smem;
   R *lf = (R *)(smem + 8 * p.max_loc);
-  double *red = smem + 8 * p.max_loc + (sizeof(R) * 3 * NPE * p.max_el + 7) / 8;
   const double dt = p.ctl->dt;
 
   for (int j = threadIdx.x; j < nloc; j += NT) {
-    int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
-    double *d = lnode + 8 * j;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      d[c] = p.x[c][g];
-      d[3 + c] = p.v[c][g];
-    }
+    const int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
+    stage_node(lnode + 8 * j, p.xm[g], p.vv[g]);
   }
   __syncthreads();
 
   EAcc ea;
   for (int t = threadIdx.x; t < nel; t += NT) {
     const int e = es + t;
-    double xs[NPE][3], vs[NPE][3];
-#pragma unroll
-    for (int k = 0; k < NPE; k++) {
-      const int l = p.conn_local[(long)e * NPE + k];
-      const double *d = lnode + 8 * l;
+    const uint16_t *lc = p.conn_local + (long)e * NPE;
+    const uint16_t *pos = p.csr_pos + (long)e * NPE;
+    auto get = [&](int k, double xk[3], double vk[3]) {
+      const double *d = lnode + 8 * lc[k];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        xs[k][c] = d[c];
-        vs[k][c] = d[3 + c];
+        xk[c] = d[c];
+        vk[c] = d[4 + c];
       }
-    }
-    R f[NPE][3];
-    elem_force<NPE, RULE, R, PHYS>(p, xs, vs, dt, e, f, ea);
+    };
+    auto put = [&](int k, const R fk[3]) {
+      R *dst = lf + 3 * pos[k];
 #pragma unroll
-    for (int k = 0; k < NPE; k++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) lf[(t * NPE + k) * 3 + c] = f[k][c];
+      for (int c = 0; c < 3; c++) dst[c] = fk[c];
+    };
+    elem_core<NPE, RULE, R, PHYS>(p, get, dt, e, put, ea);
   }
   __syncthreads();
 
   const uint16_t *cp = p.csr_ptr + loff;
-  const uint16_t *ci = p.csr_idx + coff;
   for (int j = threadIdx.x; j < nloc; j += NT) {
     double f[3] = {0.0, 0.0, 0.0};
     const int s0 = cp[j], s1 = cp[j + 1];
     for (int s = s0; s < s1; s++) {
-      const R *src = lf + 3 * ci[s];
-      f[0] += double(src[0]);
-      f[1] += double(src[1]);
-      f[2] += double(src[2]);
+      f[0] += double(lf[3 * s + 0]);
+      f[1] += double(lf[3 * s + 1]);
+      f[2] += double(lf[3 * s + 2]);
     }
     if (j < nint) {
       const double *d = lnode + 8 * j;
-      const int g = is + j;
-      double x[3] = {d[0], d[1], d[2]}, v[3] = {d[3], d[4], d[5]};
-      node_update<AUX>(p, g, f, x, v, p.m[g], dt, ea);
+      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
+      node_update<AUX>(p, is + j, f, x, v, d[3], dt, ea);
     } else {
-      double *dst = p.partial + 3L * p.bnd_slot[boff + j - nint];
-      dst[0] = f[0];
-      dst[1] = f[1];
-      dst[2] = f[2];
+      p.partial[p.bnd_slot[boff + j - nint]] = Node4{f[0], f[1], f[2], 0.0};
     }
   }
-  double row[EP_N];
-  block_reduce(ea, red, row);
-  if (threadIdx.x == 0) write_row(p, p.epart_base + b, row);
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
+}
+
+// tet4: the plan gives every workgroup at most NT elements and NT local
+// nodes, so each thread owns one element and at most one node, and every
+// global load of the workgroup (stress state, local connectivity, the
+// contribution lists, node records, slots) is issued before the first
+// barrier.  LDS: node records [max_loc][8] doubles, corner forces
+// [max_el*4][3] R, contribution list [max_el*4] u16, reduction scratch.
+template <typename R, int PHYS, bool AUX, int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_elem_block4(Params p) {
+  extern __shared__ double smem[];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int32_t *mt = p.meta + 8 * b;
+  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
+            coff = mt[6], loff = mt[7];
+  const int nloc = nint + nbnd;
+  const int t = threadIdx.x;
+  (void)coff;
+  double *lnode = smem;
+  R *lf = (R *)(smem + 8 * p.max_loc);
+  const double dt = p.ctl->dt;
+
+  const bool has_e = t < nel;
+  const int e = es + t;
+  uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
+  PreState<R> pre;
+  if (has_e) {
+    lc4 = *(const uint2 *)(p.conn_local + 4L * e);
+    ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
+    if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, 0, pre.s, pre.ep);
+  }
+  const bool has_n = t < nloc;
+  int g = 0, c0 = 0, c1 = 0, slot = 0;
+  if (has_n) {
+    g = t < nint ? is + t : p.bnd_nodes[boff + t - nint];
+    c0 = p.csr_ptr[loff + t];
+    c1 = p.csr_ptr[loff + t + 1];
+    if (t >= nint) slot = p.bnd_slot[boff + t - nint];
+    stage_node(lnode + 8 * t, p.xm[g], p.vv[g]);
+  }
+  __syncthreads();
+
+  EAcc ea;
+  if (has_e) {
+    const int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
+                      (int)(lc4.y >> 16)};
+    const int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
+                       (int)(ps4.y >> 16)};
+    auto get = [&](int k, double xk[3], double vk[3]) {
+      const double *d = lnode + 8 * l[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        xk[c] = d[c];
+        vk[c] = d[4 + c];
+      }
+    };
+    auto put = [&](int k, const R fk[3]) {
+      R *dst = lf + 3 * ps[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) dst[c] = fk[c];
+    };
+    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+  }
+  __syncthreads();
+
+  if (has_n) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int s = c0; s < c1; s++) {
+      f[0] += double(lf[3 * s + 0]);
+      f[1] += double(lf[3 * s + 1]);
+      f[2] += double(lf[3 * s + 2]);
+    }
+    if (t < nint) {
+      const double *d = lnode + 8 * t;
+      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
+      node_update<AUX>(p, g, f, x, v, d[3], dt, ea);
+    } else {
+      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+    }
+  }
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
 }
 
 template <bool AUX>
 __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
-  __shared__ double red[4 * EP_N];
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const double dt = p.ctl->dt;
   EAcc ea;
   if (i < p.n_boundary) {
     const int g = p.n_interior + i;
+    const Node4 xm = p.xm[g], vv = p.vv[g];
     double f[3] = {0.0, 0.0, 0.0};
     for (int s = p.slot_ptr[i]; s < p.slot_ptr[i + 1]; s++) {
-      const double *src = p.partial + 3L * s;
-      f[0] += src[0];
-      f[1] += src[1];
-      f[2] += src[2];
+      const Node4 q = p.partial[s];
+      f[0] += q.x;
+      f[1] += q.y;
+      f[2] += q.z;
     }
-    double x[3], v[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      x[c] = p.x[c][g];
-      v[c] = p.v[c][g];
-    }
-    node_update<AUX>(p, g, f, x, v, p.m[g], dt, ea);
+    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
+    node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
-  double row[EP_N];
-  block_reduce(ea, red, row);
-  if (threadIdx.x == 0) {
-    write_row(p, p.epart_base + blockIdx.x, row);
-    if (blockIdx.x == 0) p.ctl->step += 1;
-  }
+  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea);
+  if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -236,80 +312,73 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, typename R, int PHYS, bool ATOMIC, int NT>
 __global__ void __launch_bounds__(NT) k_elem_direct(Params p) {
-  __shared__ double red[(NT / 64) * EP_N];
   const int t = blockIdx.x * NT + threadIdx.x;
   const double dt = p.ctl->dt;
   EAcc ea;
   if (t < p.elem_count) {
     const int e = p.elem_base + t;
     int nd[NPE];
-    double xs[NPE][3], vs[NPE][3];
 #pragma unroll
-    for (int k = 0; k < NPE; k++) {
-      nd[k] = p.conn[(long)k * p.ne + e];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        xs[k][c] = p.x[c][nd[k]];
-        vs[k][c] = p.v[c][nd[k]];
-      }
-    }
-    R f[NPE][3];
-    elem_force<NPE, RULE, R, PHYS>(p, xs, vs, dt, e, f, ea);
-#pragma unroll
-    for (int k = 0; k < NPE; k++)
+    for (int k = 0; k < NPE; k++) nd[k] = p.conn[(long)k * p.ne + e];
+    auto get = [&](int k, double xk[3], double vk[3]) {
+      const Node4 a = p.xm[nd[k]], b = p.vv[nd[k]];
+      xk[0] = a.x;
+      xk[1] = a.y;
+      xk[2] = a.z;
+      vk[0] = b.x;
+      vk[1] = b.y;
+      vk[2] = b.z;
+    };
+    auto put = [&](int k, const R fk[3]) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
         if (ATOMIC)
-          unsafeAtomicAdd(&p.f[c][nd[k]], double(f[k][c]));
+          unsafeAtomicAdd(&p.f[c][nd[k]], double(fk[c]));
         else
-          p.f[c][nd[k]] += double(f[k][c]);
+          p.f[c][nd[k]] += double(fk[c]);
       }
+    };
+    elem_core<NPE, RULE, R, PHYS>(p, get, dt, e, put, ea);
   }
-  double row[EP_N];
-  block_reduce(ea, red, row);
-  if (threadIdx.x == 0) write_row(p, p.epart_base + blockIdx.x, row);
+  wave_flush<true>(p, p.epart_base + blockIdx.x * (NT / 64), ea);
 }
 
 template <bool AUX>
 __global__ void __launch_bounds__(256) k_node_all(Params p) {
-  __shared__ double red[4 * EP_N];
   const int g = blockIdx.x * 256 + threadIdx.x;
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
-    double f[3], x[3], v[3];
+    double f[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       f[c] = p.f[c][g];
       p.f[c][g] = 0.0;
-      x[c] = p.x[c][g];
-      v[c] = p.v[c][g];
     }
-    node_update<AUX>(p, g, f, x, v, p.m[g], dt, ea);
+    const Node4 xm = p.xm[g], vv = p.vv[g];
+    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
+    node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
-  double row[EP_N];
-  block_reduce(ea, red, row);
-  if (threadIdx.x == 0) {
-    write_row(p, p.epart_base + blockIdx.x, row);
-    if (blockIdx.x == 0) p.ctl->step += 1;
-  }
+  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea);
+  if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
 }
 
 // initial update with du = 0 (f_int = 0) and the half-kick
 // (dynamics.h:454-463): a0 = wall(X0) + g; v += dt/2 a0
 __global__ void __launch_bounds__(256) k_node_init(Params p) {
-  __shared__ double red[4 * EP_N];
   const int g = blockIdx.x * 256 + threadIdx.x;
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
-    const double m = p.m[g];
-    double a[3] = {0.0, 0.0, 0.0}, ke = 0.0;
+    const Node4 xm = p.xm[g], vv = p.vv[g];
+    const double m = xm.w;
+    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
+    double a[3] = {0.0, 0.0, 0.0}, ke = 0.0, vn[3];
     if (p.wall_on && (!p.contact_mask || p.contact_mask[g])) {
 #pragma unroll
       for (int c = 0; c < 3; c++)
         if (c == p.wall_dim) {
-          double d = (p.x[c][g] - p.wall_loc) * p.wall_normal;
+          double d = (x[c] - p.wall_loc) * p.wall_normal;
           if (d < 0.0) {
             a[c] += -1 * (1 / m) * d * p.wall_nk;
             ea.pen += 0.5 * p.wall_k * d * d;
@@ -320,25 +389,29 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       a[c] += p.grav[c];
-      double v = p.v[c][g];
-      ke += v * v;
-      p.v[c][g] = v + 0.5 * dt * a[c];
-      p.acc[c][g] = a[c];
-      p.du[c][g] = 0.0;
-      p.xprev[c][g] = p.x[c][g];
+      ke += v[c] * v[c];
+      vn[c] = v[c] + 0.5 * dt * a[c];
     }
+    p.vv[g] = Node4{vn[0], vn[1], vn[2], 0.0};
+    p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
+    p.du[g] = Node4{0.0, 0.0, 0.0, 0.0};
+    p.xprev[g] = xm;
     ea.ke = 0.5 * m * ke;
   }
-  double row[EP_N];
-  block_reduce(ea, red, row);
-  if (threadIdx.x == 0) write_row(p, blockIdx.x, row);
+  wave_flush<true>(p, blockIdx.x * 4, ea);
 }
 
-// sums the partial rows in a fixed order into the energy row of this step
-__global__ void __launch_bounds__(256) k_energy_reduce(Params p) {
+// Energy rows are summed in two fixed-order stages: kEredBlocks workgroups
+// each reduce a contiguous chunk of rows into p.ered, then one wave sums
+// those and writes the energy row of this step.
+constexpr int kEredBlocks = 64;
+
+__global__ void __launch_bounds__(256) k_energy_reduce1(Params p) {
   __shared__ double red[256 * EP_N];
+  const int chunk = (p.n_epart + kEredBlocks - 1) / kEredBlocks;
+  const int r0 = blockIdx.x * chunk, r1 = min(p.n_epart, r0 + chunk);
   double v[EP_N] = {0, 0, 1e300, 0, 0, 0, 0, 0};
-  for (int r = threadIdx.x; r < p.n_epart; r += 256) {
+  for (int r = r0 + threadIdx.x; r < r1; r += 256) {
     const double *src = p.epart + (long)r * EP_N;
 #pragma unroll
     for (int k = 0; k < EP_N; k++) v[k] = (k == EP_MINDT) ? fmin(v[k], src[k]) : v[k] + src[k];
@@ -356,9 +429,18 @@ __global__ void __launch_bounds__(256) k_energy_reduce(Params p) {
       }
     __syncthreads();
   }
+  if (threadIdx.x < EP_N) p.ered[blockIdx.x * EP_N + threadIdx.x] = red[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(64) k_energy_reduce2(Params p) {
   if (threadIdx.x == 0) {
+    double s[EP_N] = {0, 0, 1e300, 0, 0, 0, 0, 0};
+    for (int b = 0; b < kEredBlocks; b++)
+      for (int k = 0; k < EP_N; k++) {
+        const double o = p.ered[b * EP_N + k];
+        s[k] = (k == EP_MINDT) ? fmin(s[k], o) : s[k] + o;
+      }
     Ctl *c = p.ctl;
-    const double *s = red;
     double row[8];
     row[0] = s[EP_KE];
     row[1] = s[EP_WINT];
@@ -386,10 +468,10 @@ __global__ void __launch_bounds__(256) k_energy_reduce(Params p) {
 }
 
 // ---------------------------------------------------------------------------
-// lumped mass
+// lumped mass (into the scratch array m, then packed into xm.w)
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, int NT>
-__global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rho, double *m) {
+__global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rho) {
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   const int32_t *mt = p.meta + 8 * b;
@@ -399,9 +481,11 @@ __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rh
   double *lnode = smem;
   double *lm = smem + 4 * p.max_loc;
   for (int j = threadIdx.x; j < nloc; j += NT) {
-    int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
-#pragma unroll
-    for (int c = 0; c < 3; c++) lnode[4 * j + c] = p.x[c][g];
+    const int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
+    const Node4 q = p.xm[g];
+    lnode[4 * j + 0] = q.x;
+    lnode[4 * j + 1] = q.y;
+    lnode[4 * j + 2] = q.z;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < nel; t += NT) {
@@ -413,33 +497,33 @@ __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rh
 #pragma unroll
       for (int c = 0; c < 3; c++) xs[k][c] = lnode[4 * l + c];
     }
-    elem_mass<NPE, RULE>(p, xs, kind, rho, me);
+    elem_mass<NPE, RULE>(xs, kind, rho, me);
 #pragma unroll
-    for (int k = 0; k < NPE; k++) lm[t * NPE + k] = me[k];
+    for (int k = 0; k < NPE; k++) lm[p.csr_pos[(long)e * NPE + k]] = me[k];
   }
   __syncthreads();
+  (void)coff;
   const uint16_t *cp = p.csr_ptr + loff;
-  const uint16_t *ci = p.csr_idx + coff;
   for (int j = threadIdx.x; j < nloc; j += NT) {
     double s = 0.0;
-    for (int q = cp[j]; q < cp[j + 1]; q++) s += lm[ci[q]];
+    for (int q = cp[j]; q < cp[j + 1]; q++) s += lm[q];
     if (j < nint)
-      m[is + j] = s;
+      p.m[is + j] = s;
     else
-      p.partial[3L * p.bnd_slot[boff + j - nint]] = s;
+      p.partial[p.bnd_slot[boff + j - nint]].x = s;
   }
 }
 
-__global__ void __launch_bounds__(256) k_mass_bnd(Params p, double *m) {
+__global__ void __launch_bounds__(256) k_mass_bnd(Params p) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= p.n_boundary) return;
   double s = 0.0;
-  for (int q = p.slot_ptr[i]; q < p.slot_ptr[i + 1]; q++) s += p.partial[3L * q];
-  m[p.n_interior + i] = s;
+  for (int q = p.slot_ptr[i]; q < p.slot_ptr[i + 1]; q++) s += p.partial[q].x;
+  p.m[p.n_interior + i] = s;
 }
 
 template <int NPE, int RULE, bool ATOMIC>
-__global__ void __launch_bounds__(256) k_mass_direct(Params p, int kind, double rho, double *m) {
+__global__ void __launch_bounds__(256) k_mass_direct(Params p, int kind, double rho) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= p.elem_count) return;
   const int e = p.elem_base + t;
@@ -448,17 +532,24 @@ __global__ void __launch_bounds__(256) k_mass_direct(Params p, int kind, double 
 #pragma unroll
   for (int k = 0; k < NPE; k++) {
     nd[k] = p.conn[(long)k * p.ne + e];
-#pragma unroll
-    for (int c = 0; c < 3; c++) xs[k][c] = p.x[c][nd[k]];
+    const Node4 q = p.xm[nd[k]];
+    xs[k][0] = q.x;
+    xs[k][1] = q.y;
+    xs[k][2] = q.z;
   }
-  elem_mass<NPE, RULE>(p, xs, kind, rho, me);
+  elem_mass<NPE, RULE>(xs, kind, rho, me);
 #pragma unroll
   for (int k = 0; k < NPE; k++) {
     if (ATOMIC)
-      unsafeAtomicAdd(&m[nd[k]], me[k]);
+      unsafeAtomicAdd(&p.m[nd[k]], me[k]);
     else
-      m[nd[k]] += me[k];
+      p.m[nd[k]] += me[k];
   }
+}
+
+__global__ void __launch_bounds__(256) k_pack_mass(Params p) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g < p.nn) p.xm[g].w = p.m[g];
 }
 
 // ---------------------------------------------------------------------------
@@ -473,9 +564,10 @@ __global__ void __launch_bounds__(256) k_elem_dt(Params p, double *out) {
     double xs[4][3];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const int n = p.conn[(long)k * p.ne + e];
-#pragma unroll
-      for (int c = 0; c < 3; c++) xs[k][c] = p.x[c][n];
+      const Node4 q = p.xm[p.conn[(long)k * p.ne + e]];
+      xs[k][0] = q.x;
+      xs[k][1] = q.y;
+      xs[k][2] = q.z;
     }
     // linear corner order: tet4 (0,1,2,3); tet10 corners (2,0,1,3)
     const int o = NPE == 4 ? 0 : 2, i1 = NPE == 4 ? 1 : 0, i2 = NPE == 4 ? 2 : 1;
@@ -488,7 +580,8 @@ __global__ void __launch_bounds__(256) k_elem_dt(Params p, double *out) {
     }
     double v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
                 e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
-    d = corner_dt<double>(e1, e2, e3, v6, p.wave_speed, NPE == 4 ? 1.0 : kTet10LengthFactor);
+    d = corner_dt<double>(e1, e2, e3, v6, p.wave_speed,
+                          NPE == 4 ? kTet4LengthFactor : kTet10LengthFactor);
   }
   d = wave_min(d);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = d;
@@ -501,21 +594,20 @@ __global__ void __launch_bounds__(256) k_elem_dt(Params p, double *out) {
 // x_n + du - X0 with B(x_n) (dynamics_kernels.h:117-130, analysis.h:479-503)
 // ---------------------------------------------------------------------------
 template <int NPE>
-__global__ void __launch_bounds__(256) k_stress_elem(Params p, const double *X0x,
-                                                      const double *X0y, const double *X0z,
-                                                      double *out) {
+__global__ void __launch_bounds__(256) k_stress_elem(Params p, double *out) {
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= p.ne) return;
-  const double *X0[3] = {X0x, X0y, X0z};
   double xs[NPE][3], u[NPE][3];
 #pragma unroll
   for (int k = 0; k < NPE; k++) {
     const int n = p.conn[(long)k * p.ne + e];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      xs[k][c] = p.xprev[c][n];
-      u[k][c] = p.du[c][n] + p.xprev[c][n] - X0[c][n];
-    }
+    const Node4 xp = p.xprev[n], d = p.du[n], x0 = p.X0[n];
+    xs[k][0] = xp.x;
+    xs[k][1] = xp.y;
+    xs[k][2] = xp.z;
+    u[k][0] = d.x + xp.x - x0.x;
+    u[k][1] = d.y + xp.y - x0.y;
+    u[k][2] = d.z + xp.z - x0.z;
   }
   double G[NPE * 3];
   if constexpr (NPE == 4) {
@@ -560,8 +652,9 @@ __global__ void __launch_bounds__(256) k_stress_elem(Params p, const double *X0x
 }
 
 // ---------------------------------------------------------------------------
-// known-answer strain energy and volume (dynamics.h:303-412,
-// analysis.h:384-477): W_e = 1/2 sum_q w detJ de.D.de, V_e = sum_q w detJ
+// known-answer strain energy and volume on the reference positions
+// (dynamics.h:303-412, analysis.h:384-477): W_e = 1/2 sum_q w detJ de.D.de,
+// V_e = sum_q w detJ
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE>
 __global__ void __launch_bounds__(256) k_debug_strain(Params p, double alpha, int dcase,
@@ -573,18 +666,18 @@ __global__ void __launch_bounds__(256) k_debug_strain(Params p, double alpha, in
     double xs[NPE][3], u[NPE][3];
 #pragma unroll
     for (int k = 0; k < NPE; k++) {
-      const int n = p.conn[(long)k * p.ne + e];
-#pragma unroll
-      for (int c = 0; c < 3; c++) xs[k][c] = p.x[c][n];
-      const double x = xs[k][0];
+      const Node4 q = p.X0[p.conn[(long)k * p.ne + e]];
+      xs[k][0] = q.x;
+      xs[k][1] = q.y;
+      xs[k][2] = q.z;
+      const double x = q.x;
       const double s = dcase == 0 ? alpha * x : alpha * 0.5 * x * x;
       u[k][0] = s;
       u[k][1] = -s * nu;
       u[k][2] = -s * nu;
     }
-    constexpr int NQ = NPE == 4 ? Quad<RULE>::n : Quad<RULE>::n;
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
+    for (int q = 0; q < Quad<RULE>::n; q++) {
       double G[NPE * 3];
       if constexpr (NPE == 4) {
         const double g4[12] = {-1, -1, -1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -640,21 +733,36 @@ __global__ void __launch_bounds__(256) k_debug_strain(Params p, double alpha, in
 }
 
 // ---------------------------------------------------------------------------
+// streaming copy (bandwidth probe and counter calibration)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_stream_copy(const Node4 *__restrict__ src,
+                                                     Node4 *__restrict__ dst, long n) {
+  const long stride = (long)gridDim.x * 256;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 
-#define CPI_LAUNCH(kern, grid, block, smem, stream, ...)                       \
-  do {                                                                          \
-    hipLaunchKernelGGL((kern), dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__); \
-    return hipGetLastError();                                                   \
+#define CPI_LAUNCH(kern, grid, block, smem, stream, ...)                                 \
+  do {                                                                                   \
+    hipLaunchKernelGGL((kern), dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__);  \
+    return hipGetLastError();                                                            \
   } while (0)
 
-template <int NPE> constexpr int block_threads() { return NPE == 4 ? 512 : 128; }
+template <int NPE> constexpr int block_threads() { return NPE == 4 ? 384 : 128; }
+
+// minimum waves per SIMD the tet4 element kernel is compiled for: FP64 fits
+// 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU)
+template <typename R> constexpr int tet4_waves() { return std::is_same<R, double>::value ? 5 : 8; }
 
 size_t elem_block_smem(int npe, int prec, int max_loc, int max_el) {
   const size_t rsz = prec == 1 ? 4 : 8;
   const int nt = npe == 4 ? block_threads<4>() : block_threads<10>();
-  return 8 * 8 * (size_t)max_loc + ((rsz * 3 * npe * max_el + 7) / 8) * 8 + 8 * EP_N * (nt / 64);
+  size_t s = 8 * 8 * (size_t)max_loc + ((rsz * 3 * npe * max_el + 7) / 8) * 8 + 8 * EP_N * (nt / 64);
+  if (npe == 4) s += 8 * (size_t)max_el;  // staged contribution lists
+  return s;
 }
 
 int elem_block_threads(int npe) { return npe == 4 ? block_threads<4>() : block_threads<10>(); }
@@ -672,34 +780,42 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
 template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
   constexpr int NT = block_threads<NPE>();
-  if (aux) CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, true, NT>), nblocks, NT, smem, s, p);
-  CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, false, NT>), nblocks, NT, smem, s, p);
+  if constexpr (NPE == 4) {
+    constexpr int W = tet4_waves<R>();
+    if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
+  } else {
+    if (aux) CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, true, NT>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, false, NT>), nblocks, NT, smem, s, p);
+  }
 }
 
 template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_direct_t(const Params &p, bool atomic, hipStream_t s) {
-  constexpr int NT = NPE == 4 ? 256 : 64;
+  constexpr int NT = NPE == 4 ? 256 : 128;
   const int grid = (p.elem_count + NT - 1) / NT;
   if (grid == 0) return hipSuccess;
   if (atomic) CPI_LAUNCH((k_elem_direct<NPE, RULE, R, PHYS, true, NT>), grid, NT, 0, s, p);
   CPI_LAUNCH((k_elem_direct<NPE, RULE, R, PHYS, false, NT>), grid, NT, 0, s, p);
 }
 
+int elem_direct_threads(int npe) { return npe == 4 ? 256 : 128; }
+
 // (npe, rule, precision, physics) -> instantiation
 template <template <int, int, typename, int> class F, typename... A>
 static hipError_t dispatch(int npe, int rule, int prec, int phys, A &&...a) {
-#define CPI_PHYS(NPE, RULE, R)                                              \
-  switch (phys) {                                                           \
-    case PHYS_COMPAT: return F<NPE, RULE, R, PHYS_COMPAT>::run(a...);      \
-    case PHYS_EP: return F<NPE, RULE, R, PHYS_EP>::run(a...);              \
-    case PHYS_EP_MLP: return F<NPE, RULE, R, PHYS_EP_MLP>::run(a...);      \
-  }                                                                         \
+#define CPI_PHYS(NPE, RULE, R)                                         \
+  switch (phys) {                                                      \
+    case PHYS_COMPAT: return F<NPE, RULE, R, PHYS_COMPAT>::run(a...); \
+    case PHYS_EP: return F<NPE, RULE, R, PHYS_EP>::run(a...);         \
+    case PHYS_EP_MLP: return F<NPE, RULE, R, PHYS_EP_MLP>::run(a...); \
+  }                                                                    \
   return hipErrorInvalidValue;
-#define CPI_PREC(NPE, RULE)           \
-  if (prec == 1) {                    \
-    CPI_PHYS(NPE, RULE, float)        \
-  } else {                            \
-    CPI_PHYS(NPE, RULE, double)       \
+#define CPI_PREC(NPE, RULE)    \
+  if (prec == 1) {             \
+    CPI_PHYS(NPE, RULE, float) \
+  } else {                     \
+    CPI_PHYS(NPE, RULE, double) \
   }
   if (npe == 4) {
     CPI_PREC(4, RULE_CENTROID)
@@ -714,9 +830,14 @@ static hipError_t dispatch(int npe, int rule, int prec, int phys, A &&...a) {
   return hipErrorInvalidValue;
 }
 
+template <int NPE, typename R, int PHYS>
+constexpr bool valid_combo() {
+  return PHYS != PHYS_EP_MLP || (NPE == 4 && std::is_same<R, float>::value);
+}
+
 template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockF {
   static hipError_t run(const Params &p, bool aux, int nb, size_t smem, hipStream_t s) {
-    if constexpr (PHYS == PHYS_EP_MLP && (NPE != 4 || !std::is_same<R, float>::value))
+    if constexpr (!valid_combo<NPE, R, PHYS>())
       return hipErrorInvalidValue;
     else
       return elem_block_t<NPE, RULE, R, PHYS>(p, aux, nb, smem, s);
@@ -724,13 +845,20 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockF {
 };
 template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
   static hipError_t run(size_t smem) {
-    if constexpr (PHYS == PHYS_EP_MLP && (NPE != 4 || !std::is_same<R, float>::value)) {
+    if constexpr (!valid_combo<NPE, R, PHYS>()) {
       return hipErrorInvalidValue;
     } else {
       constexpr int NT = block_threads<NPE>();
-      hipError_t e = lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, true, NT>, smem);
-      if (e != hipSuccess) return e;
-      return lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, false, NT>, smem);
+      if constexpr (NPE == 4) {
+        constexpr int W = tet4_waves<R>();
+        hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
+        if (e != hipSuccess) return e;
+        return lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
+      } else {
+        hipError_t e = lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, true, NT>, smem);
+        if (e != hipSuccess) return e;
+        return lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, false, NT>, smem);
+      }
     }
   }
 };
@@ -774,83 +902,52 @@ hipError_t launch_node_init(const Params &p, hipStream_t s) {
 }
 
 hipError_t launch_energy_reduce(const Params &p, hipStream_t s) {
-  CPI_LAUNCH(k_energy_reduce, 1, 256, 0, s, p);
+  hipLaunchKernelGGL(k_energy_reduce1, dim3(kEredBlocks), dim3(256), 0, s, p);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  CPI_LAUNCH(k_energy_reduce2, 1, 64, 0, s, p);
 }
 
-hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho, double *m,
-                       int nblocks, hipStream_t s) {
+int energy_reduce_blocks() { return kEredBlocks; }
+
+template <int NPE, int RULE>
+static hipError_t mass_rule_t(const Params &p, int scatter, int kind, double rho, int nblocks,
+                              hipStream_t s) {
   if (scatter == 0) {
-    const int nt = p.npe == 4 ? 512 : 128;
-    const size_t smem = 8 * (4 * (size_t)p.max_loc + (size_t)p.max_el * p.npe);
+    constexpr int NT = NPE == 4 ? 512 : 128;
+    const size_t smem = 8 * (4 * (size_t)p.max_loc + (size_t)p.max_el * NPE);
     if (smem > 65536) return hipErrorInvalidValue;  // sized by the plan to stay below
-    hipError_t err;
-    if (p.npe == 4) {
-      switch (rule) {
-        case RULE_REF4:
-          hipLaunchKernelGGL((k_mass_block<4, RULE_REF4, 512>), dim3(nblocks), dim3(nt), smem, s,
-                             p, kind, rho, m);
-          break;
-        case RULE_EXACT4:
-          hipLaunchKernelGGL((k_mass_block<4, RULE_EXACT4, 512>), dim3(nblocks), dim3(nt), smem,
-                             s, p, kind, rho, m);
-          break;
-        case RULE_CENTROID:
-          hipLaunchKernelGGL((k_mass_block<4, RULE_CENTROID, 512>), dim3(nblocks), dim3(nt), smem,
-                             s, p, kind, rho, m);
-          break;
-        default:
-          hipLaunchKernelGGL((k_mass_block<4, RULE_REF5, 512>), dim3(nblocks), dim3(nt), smem, s,
-                             p, kind, rho, m);
-      }
-    } else {
-      switch (rule) {
-        case RULE_REF5:
-          hipLaunchKernelGGL((k_mass_block<10, RULE_REF5, 128>), dim3(nblocks), dim3(nt), smem, s,
-                             p, kind, rho, m);
-          break;
-        case RULE_REF4:
-          hipLaunchKernelGGL((k_mass_block<10, RULE_REF4, 128>), dim3(nblocks), dim3(nt), smem, s,
-                             p, kind, rho, m);
-          break;
-        default:
-          hipLaunchKernelGGL((k_mass_block<10, RULE_EXACT4, 128>), dim3(nblocks), dim3(nt), smem,
-                             s, p, kind, rho, m);
-      }
-    }
-    if ((err = hipGetLastError()) != hipSuccess) return err;
-    if (p.n_boundary > 0)
-      hipLaunchKernelGGL(k_mass_bnd, dim3((p.n_boundary + 255) / 256), dim3(256), 0, s, p, m);
-    return hipGetLastError();
+    hipLaunchKernelGGL((k_mass_block<NPE, RULE, NT>), dim3(nblocks), dim3(NT), smem, s, p, kind,
+                       rho);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || p.n_boundary == 0) return e;
+    CPI_LAUNCH(k_mass_bnd, (p.n_boundary + 255) / 256, 256, 0, s, p);
   }
   const int grid = (p.elem_count + 255) / 256;
   if (grid == 0) return hipSuccess;
-  const bool at = scatter == 2;
-#define CPI_MD(NPE, RULE)                                                                        \
-  if (at)                                                                                        \
-    hipLaunchKernelGGL((k_mass_direct<NPE, RULE, true>), dim3(grid), dim3(256), 0, s, p, kind, rho, m); \
-  else                                                                                           \
-    hipLaunchKernelGGL((k_mass_direct<NPE, RULE, false>), dim3(grid), dim3(256), 0, s, p, kind, rho, m);
+  if (scatter == 2) CPI_LAUNCH((k_mass_direct<NPE, RULE, true>), grid, 256, 0, s, p, kind, rho);
+  CPI_LAUNCH((k_mass_direct<NPE, RULE, false>), grid, 256, 0, s, p, kind, rho);
+}
+
+hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho, int nblocks,
+                       hipStream_t s) {
   if (p.npe == 4) {
-    if (rule == RULE_CENTROID) {
-      CPI_MD(4, RULE_CENTROID)
-    } else if (rule == RULE_REF4) {
-      CPI_MD(4, RULE_REF4)
-    } else if (rule == RULE_EXACT4) {
-      CPI_MD(4, RULE_EXACT4)
-    } else {
-      CPI_MD(4, RULE_REF5)
-    }
-  } else {
-    if (rule == RULE_REF4) {
-      CPI_MD(10, RULE_REF4)
-    } else if (rule == RULE_EXACT4) {
-      CPI_MD(10, RULE_EXACT4)
-    } else {
-      CPI_MD(10, RULE_REF5)
+    switch (rule) {
+      case RULE_REF4: return mass_rule_t<4, RULE_REF4>(p, scatter, kind, rho, nblocks, s);
+      case RULE_EXACT4: return mass_rule_t<4, RULE_EXACT4>(p, scatter, kind, rho, nblocks, s);
+      case RULE_CENTROID: return mass_rule_t<4, RULE_CENTROID>(p, scatter, kind, rho, nblocks, s);
+      default: return mass_rule_t<4, RULE_REF5>(p, scatter, kind, rho, nblocks, s);
     }
   }
-#undef CPI_MD
-  return hipGetLastError();
+  switch (rule) {
+    case RULE_REF4: return mass_rule_t<10, RULE_REF4>(p, scatter, kind, rho, nblocks, s);
+    case RULE_EXACT4: return mass_rule_t<10, RULE_EXACT4>(p, scatter, kind, rho, nblocks, s);
+    default: return mass_rule_t<10, RULE_REF5>(p, scatter, kind, rho, nblocks, s);
+  }
+}
+
+hipError_t launch_pack_mass(const Params &p, hipStream_t s) {
+  CPI_LAUNCH(k_pack_mass, (p.nn + 255) / 256, 256, 0, s, p);
 }
 
 hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s) {
@@ -859,29 +956,36 @@ hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s) {
   CPI_LAUNCH(k_elem_dt<10>, grid, 256, 0, s, p, out);
 }
 
-hipError_t launch_stress_elem(const Params &p, const double *const X0[3], double *out,
-                              hipStream_t s) {
+hipError_t launch_stress_elem(const Params &p, double *out, hipStream_t s) {
   const int grid = (p.ne + 255) / 256;
-  if (p.npe == 4) CPI_LAUNCH(k_stress_elem<4>, grid, 256, 0, s, p, X0[0], X0[1], X0[2], out);
-  CPI_LAUNCH(k_stress_elem<10>, grid, 256, 0, s, p, X0[0], X0[1], X0[2], out);
+  if (p.npe == 4) CPI_LAUNCH(k_stress_elem<4>, grid, 256, 0, s, p, out);
+  CPI_LAUNCH(k_stress_elem<10>, grid, 256, 0, s, p, out);
 }
 
 hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcase, double nu,
                                double *out, hipStream_t s) {
   const int grid = (p.ne + 255) / 256;
+#define CPI_DS(NPE, RULE) \
+  CPI_LAUNCH((k_debug_strain<NPE, RULE>), grid, 256, 0, s, p, alpha, dcase, nu, out)
   if (p.npe == 4) {
     switch (rule) {
-      case RULE_REF4: CPI_LAUNCH((k_debug_strain<4, RULE_REF4>), grid, 256, 0, s, p, alpha, dcase, nu, out);
-      case RULE_EXACT4: CPI_LAUNCH((k_debug_strain<4, RULE_EXACT4>), grid, 256, 0, s, p, alpha, dcase, nu, out);
-      case RULE_CENTROID: CPI_LAUNCH((k_debug_strain<4, RULE_CENTROID>), grid, 256, 0, s, p, alpha, dcase, nu, out);
-      default: CPI_LAUNCH((k_debug_strain<4, RULE_REF5>), grid, 256, 0, s, p, alpha, dcase, nu, out);
+      case RULE_REF4: CPI_DS(4, RULE_REF4);
+      case RULE_EXACT4: CPI_DS(4, RULE_EXACT4);
+      case RULE_CENTROID: CPI_DS(4, RULE_CENTROID);
+      default: CPI_DS(4, RULE_REF5);
     }
   }
   switch (rule) {
-    case RULE_REF4: CPI_LAUNCH((k_debug_strain<10, RULE_REF4>), grid, 256, 0, s, p, alpha, dcase, nu, out);
-    case RULE_EXACT4: CPI_LAUNCH((k_debug_strain<10, RULE_EXACT4>), grid, 256, 0, s, p, alpha, dcase, nu, out);
-    default: CPI_LAUNCH((k_debug_strain<10, RULE_REF5>), grid, 256, 0, s, p, alpha, dcase, nu, out);
+    case RULE_REF4: CPI_DS(10, RULE_REF4);
+    case RULE_EXACT4: CPI_DS(10, RULE_EXACT4);
+    default: CPI_DS(10, RULE_REF5);
   }
+#undef CPI_DS
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, long n4, hipStream_t s) {
+  const int grid = (int)std::min<long>((n4 + 255) / 256, 8L * 1024);
+  CPI_LAUNCH(k_stream_copy, grid, 256, 0, s, (const Node4 *)src, (Node4 *)dst, n4);
 }
 
 }  // namespace cpi

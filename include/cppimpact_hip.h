@@ -242,6 +242,12 @@ int cpi_profile_kernels(cpi_ctx *ctx, int32_t nsteps, cpi_kernel_time *out,
 int cpi_debug_strain(cpi_ctx *ctx, double alpha, int32_t def_case,
                      double *energy, double *volume);
 
+/* STREAM-style copy of bytes (read + write counted) repeated reps times on
+   device; returns the achieved GB/s.  No counterpart in the reference: it
+   measures the HBM bandwidth the roofline is compared against and
+   calibrates the memory counters (a copy moves a known number of bytes). */
+int cpi_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs);
+
 /* ---- host-only utilities (no GPU needed) -------------------------------- */
 
 /* Greedy colouring in element order, smallest free colour; returns the
