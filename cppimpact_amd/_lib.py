@@ -89,6 +89,16 @@ class InpCounts(C.Structure):
         "num_slave_nodes", "names_bytes")]
 
 
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("lr_final", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("seed", C.c_uint64)]
+
+
+MLP_INPUTS, MLP_OUTPUTS, MLP_NPARAMS, SAMPLE_FLOATS = 13, 7, 5511, 20
+MLP_ELASTIC, MLP_ACCEPTED, MLP_FALLBACK = 0, 1, 2
+_fp = C.POINTER(C.c_float)
+
 # every exported symbol and its (restype, argtypes)
 SIGNATURES = {
     "cpi_default_options": (None, [C.POINTER(Options)]),
@@ -119,6 +129,29 @@ SIGNATURES = {
     "cpi_inp_read": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(InpCounts), _dp, _ip, _ip, _ip,
                                _ip, _ip, C.c_char_p]),
     "cpi_inp_write": (C.c_int, [C.c_char_p, C.POINTER(MeshDesc)]),
+    "cpi_mlp_default_train_opts": (None, [C.POINTER(MlpTrainOpts)]),
+    "cpi_mlp_create": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(Material),
+                                 C.POINTER(C.c_void_p)]),
+    "cpi_mlp_destroy": (None, [C.c_void_p]),
+    "cpi_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "cpi_mlp_set_params": (C.c_int, [C.c_void_p, _fp]),
+    "cpi_mlp_get_params": (C.c_int, [C.c_void_p, _fp]),
+    "cpi_mlp_set_scaling": (C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp]),
+    "cpi_mlp_get_scaling": (C.c_int, [C.c_void_p, _fp, _fp, _fp, _fp]),
+    "cpi_mlp_fit_scaling": (C.c_int, [C.c_void_p, _fp, C.c_int32]),
+    "cpi_mlp_targets": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp, _fp, _ip]),
+    "cpi_mlp_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "cpi_mlp_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]),
+    "cpi_mlp_train": (C.c_int, [C.c_void_p, _fp, C.c_int32, C.POINTER(MlpTrainOpts), _fp]),
+    "cpi_mlp_return_map": (C.c_int, [C.c_void_p, C.c_double, C.c_int32, _fp, _fp, _fp, _fp,
+                                     _fp, _ip]),
+    "cpi_set_surrogate": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cpi_get_surrogate_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64)]),
+    "cpi_reset_surrogate_stats": (C.c_int, [C.c_void_p]),
+    "cpi_samples_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "cpi_samples_get": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip]),
+    "cpi_samples_reset": (C.c_int, [C.c_void_p]),
 }
 
 _LIB = None
@@ -149,9 +182,20 @@ def iptr(a):
     return a.ctypes.data_as(_ip) if a is not None else None
 
 
+def fptr(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
 def check(rc, ctx=None):
     if rc < 0:
         msg = lib().cpi_last_error(ctx)
+        raise CpiError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def check_mlp(rc, m=None):
+    if rc < 0:
+        msg = lib().cpi_mlp_last_error(m)
         raise CpiError(rc, msg.decode() if msg else "")
     return rc
 

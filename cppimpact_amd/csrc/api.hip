@@ -398,7 +398,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     const size_t rsz = c->prec == CPI_PREC_F32 ? 4 : 8;
     const long share = npe == 4 ? (c->prec == CPI_PREC_F32 ? 40960 : 54272) : 81920;
     const long budget = std::min<long>((long)prop.sharedMemPerBlock, share);
-    const long fixed = (long)elem_block_smem(npe, c->prec, 0, me);
+    const long fixed = (long)elem_block_smem(npe, c->prec, c->phys, 0, me);
     long mnl = (budget - fixed - 64) / 64;
     if (mnl < 4L * npe)
       return fail(c, CPI_ERR_INVALID, "block_elements " + std::to_string(me) +
@@ -515,7 +515,8 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     P.n_boundary = B.n_boundary;
     P.max_loc = B.max_local_nodes;
     P.max_el = B.max_block_elems;
-    c->smem = elem_block_smem(npe, c->prec, P.max_loc, P.max_el);
+    c->smem = elem_block_smem(npe, c->prec, c->phys, P.max_loc, P.max_el);
+    P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.max_loc, P.max_el);
     if (c->smem > (size_t)prop.sharedMemPerBlock)
       return fail(c, CPI_ERR_INVALID, "element block needs " + std::to_string(c->smem) +
                                           " bytes of LDS; lower block_elements");
@@ -633,6 +634,8 @@ extern "C" int cpi_stable_dt(cpi_ctx *c, double *dte, double *dtc) {
 extern "C" int cpi_init(cpi_ctx *c, double dt) {
   if (!c) return CPI_ERR_INVALID;
   if (!std::isfinite(dt)) return fail(c, CPI_ERR_INVALID, "cpi_init: dt must be finite");
+  if (c->phys == 2 && !c->mlp.ready)
+    return fail(c, CPI_ERR_STATE, "cpi_init: CPI_RETURN_MLP needs cpi_set_surrogate first");
   drop_graphs(c);
   c->inited = false;
   int rc = reset_state(c);
@@ -976,4 +979,124 @@ extern "C" int cpi_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, 
   if (a) (void)hipFree(a);
   if (b) (void)hipFree(b);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// surrogate installation, counters and return-map samples
+// ---------------------------------------------------------------------------
+
+namespace {
+
+bool same_value(double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)); }
+
+// re-captures the step graphs after a change of the kernel arguments
+int refresh_graphs(cpi_ctx *c) {
+  drop_graphs(c);
+  if (c->inited && c->opt.use_graph) {
+    int rc;
+    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux))) return rc;
+  }
+  return CPI_OK;
+}
+
+}  // namespace
+
+extern "C" int cpi_set_surrogate(cpi_ctx *c, const cpi_mlp *m) {
+  if (!c || !m) return CPI_ERR_INVALID;
+  if (c->phys != 2)
+    return fail(c, CPI_ERR_STATE, "cpi_set_surrogate: the context was not created with CPI_RETURN_MLP");
+  if (m->dev != c->dev) return fail(c, CPI_ERR_INVALID, "cpi_set_surrogate: network is on another device");
+  if (!m->scaled) return fail(c, CPI_ERR_STATE, "cpi_set_surrogate: network has no scaling (train or set it)");
+  const MlpMaterial a = mlp_material(c->mat), &b = m->mm;
+  if (!same_value(a.c11, b.c11) || !same_value(a.c12, b.c12) || !same_value(a.c44, b.c44) ||
+      !same_value(a.sy0, b.sy0) || !same_value(a.H, b.H))
+    return fail(c, CPI_ERR_INVALID, "cpi_set_surrogate: network was made for another material");
+  HIPCHK(c, hipSetDevice(c->dev));
+  HIPCHK(c, hipStreamSynchronize(m->stream));
+  HIPCHK(c, hipMemcpyAsync(c->mlp.d_weights, m->W, sizeof(MlpWeights), hipMemcpyDeviceToDevice, c->stream));
+  MlpParams h{};
+  h.w1 = c->mlp.d_weights->w1;
+  h.w2p = c->mlp.d_weights->w2p;
+  h.w3p = c->mlp.d_weights->w3p;
+  h.b1 = c->mlp.d_weights->b1;
+  h.b2 = c->mlp.d_weights->b2;
+  h.b3 = c->mlp.d_weights->b3;
+  for (int k = 0; k < kMlpIn; k++) {
+    h.in_mean[k] = m->in_mean[k];
+    h.in_istd[k] = 1.0f / m->in_std[k];
+  }
+  for (int k = 0; k < kMlpOut; k++) {
+    h.out_mean[k] = m->out_mean[k];
+    h.out_std[k] = m->out_std[k];
+  }
+  h.c11 = (float)a.c11;
+  h.c12 = (float)a.c12;
+  h.c44 = (float)a.c44;
+  h.sy0 = (float)a.sy0;
+  h.H = (float)a.H;
+  HIPCHK(c, hipMemcpyAsync(c->mlp.d_params, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mlp.ready = true;
+  return CPI_OK;
+}
+
+extern "C" int cpi_get_surrogate_stats(cpi_ctx *c, int64_t *accepted, int64_t *fallback) {
+  if (!c) return CPI_ERR_INVALID;
+  unsigned long long h[2] = {0, 0};
+  if (c->mlp.d_stats) {
+    HIPCHK(c, hipMemcpyAsync(h, c->mlp.d_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (accepted) *accepted = (int64_t)h[0];
+  if (fallback) *fallback = (int64_t)h[1];
+  return CPI_OK;
+}
+
+extern "C" int cpi_reset_surrogate_stats(cpi_ctx *c) {
+  if (!c) return CPI_ERR_INVALID;
+  if (c->mlp.d_stats) {
+    HIPCHK(c, hipMemsetAsync(c->mlp.d_stats, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return CPI_OK;
+}
+
+extern "C" int cpi_samples_enable(cpi_ctx *c, int32_t cap) {
+  if (!c || cap < 0) return CPI_ERR_INVALID;
+  if (c->phys != CPI_PHYSICS_EP)
+    return fail(c, CPI_ERR_STATE, "cpi_samples_enable: needs CPI_PHYSICS_EP with the exact return");
+  HIPCHK(c, hipSetDevice(c->dev));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = mlp_samples_alloc(c->mlp, cap, c->err);
+  if (rc) return rc;
+  c->P.samples = c->mlp.d_samples;
+  c->P.sample_count = c->mlp.d_sample_count;
+  c->P.sample_cap = c->mlp.sample_cap;
+  return refresh_graphs(c);
+}
+
+extern "C" int cpi_samples_get(cpi_ctx *c, float *out, int32_t cap, int32_t *recorded) {
+  if (!c || cap < 0 || (cap > 0 && !out)) return CPI_ERR_INVALID;
+  int cnt = 0;
+  if (c->mlp.d_sample_count) {
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->mlp.d_sample_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  const int k = std::min(std::min(cnt, c->mlp.sample_cap), cap);
+  if (k > 0) {
+    HIPCHK(c, hipMemcpyAsync(out, c->mlp.d_samples, sizeof(float) * kSampleFloats * (size_t)k,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (recorded) *recorded = cnt;
+  return CPI_OK;
+}
+
+extern "C" int cpi_samples_reset(cpi_ctx *c) {
+  if (!c) return CPI_ERR_INVALID;
+  if (c->mlp.d_sample_count) {
+    HIPCHK(c, hipMemsetAsync(c->mlp.d_sample_count, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return CPI_OK;
 }

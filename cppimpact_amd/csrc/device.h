@@ -89,10 +89,11 @@ struct Params {
   int elem_count;
   const MlpParams *mlp;  // null unless the surrogate is enabled
   double mlp_tol;        // fallback threshold on |f_yield| / sy0
-  int *mlp_stats;        // [2] surrogate accepted / fell back (optional)
+  unsigned long long *mlp_stats;  // [2] surrogate accepted / fell back (optional)
   float *samples;        // return-mapping sample buffer (optional)
   int *sample_count;
   int sample_cap;
+  int mlp_stage_offset;  // byte offset of the per-wave MLP staging in LDS
 };
 
 // ---------------------------------------------------------------------------

@@ -75,11 +75,13 @@ template <typename R> struct PreState {
 // Stress at one point from the strain increment de.  compat: sigma = D de
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
-// point's internal work and plastic dissipation (times wd) to ea.
+// point's internal work and plastic dissipation (times wd) to ea.  An
+// inactive point (a surrogate lane with no element) stores nothing.
 template <typename R, int PHYS, bool PRE = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
-                                             const PreState<R> *pre = nullptr) {
+                                             const PreState<R> *pre = nullptr,
+                                             bool active = true) {
   const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
   R ds[6];
   ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
@@ -115,7 +117,7 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     }
     R dg;
     if constexpr (PHYS == PHYS_EP_MLP) {
-      dg = mlp_return<R>(p, de, so, ep, s);
+      dg = mlp_return<R>(p, de, so, ep, s, active);
     } else {
       R ep0 = ep;
       dg = radial_return<R>(p, s, ep);
@@ -126,7 +128,7 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-    store_state<R>(p, e, q, s, ep);
+    if (active) store_state<R>(p, e, q, s, ep);
   }
 }
 
@@ -169,7 +171,7 @@ constexpr double kTet10LengthFactor = 4.35;
 template <typename R, int PHYS, bool PRE, class Get, class Put>
 __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
                                           const Put &put, EAcc &ea,
-                                          const PreState<R> *pre = nullptr) {
+                                          const PreState<R> *pre = nullptr, bool active = true) {
   R J[9];
   R u[3][3];
   {
@@ -205,7 +207,7 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
              L[0][2] + L[2][0]};
   R wd = R(p.wsum) * detJ;
   R s[6];
-  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre);
+  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre, active);
   R f0[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int k = 1; k < 4; k++) {

@@ -140,7 +140,7 @@ __device__ __forceinline__ void stage_node(double *d, const Node4 &xm, const Nod
 // ---------------------------------------------------------------------------
 // LDS-assembled element pass, generic (tet10): loops over elements and
 // nodes.  LDS: node records [max_loc][8] doubles, corner forces
-// [max_el*NPE][3] R, reduction scratch.
+// [max_el*NPE][3] R.
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, typename R, int PHYS, bool AUX, int NT>
 __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
@@ -208,7 +208,10 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 // global load of the workgroup (stress state, local connectivity, the
 // contribution lists, node records, slots) is issued before the first
 // barrier.  LDS: node records [max_loc][8] doubles, corner forces
-// [max_el*4][3] R, contribution list [max_el*4] u16, reduction scratch.
+// [max_el*4][3] R, and with the surrogate one MLP staging area per wave.
+// The surrogate's return map is wave-cooperative, so with it every lane runs
+// the element code: lanes past the block's last element repeat its first
+// element and store nothing.
 template <typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
@@ -224,11 +227,12 @@ k_elem_block4(Params p) {
   R *lf = (R *)(smem + 8 * p.max_loc);
   const double dt = p.ctl->dt;
 
+  constexpr bool UNIFORM = PHYS == PHYS_EP_MLP;
   const bool has_e = t < nel;
-  const int e = es + t;
+  const int e = has_e ? es + t : es;
   uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
   PreState<R> pre;
-  if (has_e) {
+  if (UNIFORM || has_e) {
     lc4 = *(const uint2 *)(p.conn_local + 4L * e);
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
     if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, 0, pre.s, pre.ep);
@@ -245,7 +249,7 @@ k_elem_block4(Params p) {
   __syncthreads();
 
   EAcc ea;
-  if (has_e) {
+  if (UNIFORM || has_e) {
     const int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
                       (int)(lc4.y >> 16)};
     const int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
@@ -259,11 +263,13 @@ k_elem_block4(Params p) {
       }
     };
     auto put = [&](int k, const R fk[3]) {
+      if (UNIFORM && !has_e) return;
       R *dst = lf + 3 * ps[k];
 #pragma unroll
       for (int c = 0; c < 3; c++) dst[c] = fk[c];
     };
-    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre, has_e);
+    if (UNIFORM && !has_e) ea = EAcc();
   }
   __syncthreads();
 
@@ -754,14 +760,23 @@ __global__ void __launch_bounds__(256) k_stream_copy(const Node4 *__restrict__ s
 template <int NPE> constexpr int block_threads() { return NPE == 4 ? 384 : 128; }
 
 // minimum waves per SIMD the tet4 element kernel is compiled for: FP64 fits
-// 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU)
-template <typename R> constexpr int tet4_waves() { return std::is_same<R, double>::value ? 5 : 8; }
+// 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU); the
+// surrogate's inference needs 128 (two per CU)
+template <typename R, int PHYS> constexpr int tet4_waves() {
+  return PHYS == PHYS_EP_MLP ? 4 : std::is_same<R, double>::value ? 5 : 8;
+}
 
-size_t elem_block_smem(int npe, int prec, int max_loc, int max_el) {
+// LDS of the workgroup element kernels: node records, corner forces and,
+// with the surrogate, one MLP staging area per wave
+size_t elem_block_mlp_offset(int npe, int prec, int max_loc, int max_el) {
   const size_t rsz = prec == 1 ? 4 : 8;
+  return 8 * 8 * (size_t)max_loc + ((rsz * 3 * npe * max_el + 15) / 16) * 16;
+}
+
+size_t elem_block_smem(int npe, int prec, int phys, int max_loc, int max_el) {
   const int nt = npe == 4 ? block_threads<4>() : block_threads<10>();
-  size_t s = 8 * 8 * (size_t)max_loc + ((rsz * 3 * npe * max_el + 7) / 8) * 8 + 8 * EP_N * (nt / 64);
-  if (npe == 4) s += 8 * (size_t)max_el;  // staged contribution lists
+  size_t s = elem_block_mlp_offset(npe, prec, max_loc, max_el);
+  if (phys == PHYS_EP_MLP) s += (size_t)kMlpStageBytes * (nt / 64);
   return s;
 }
 
@@ -781,7 +796,7 @@ template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
   constexpr int NT = block_threads<NPE>();
   if constexpr (NPE == 4) {
-    constexpr int W = tet4_waves<R>();
+    constexpr int W = tet4_waves<R, PHYS>();
     if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
     CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
   } else {
@@ -850,7 +865,7 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
     } else {
       constexpr int NT = block_threads<NPE>();
       if constexpr (NPE == 4) {
-        constexpr int W = tet4_waves<R>();
+        constexpr int W = tet4_waves<R, PHYS>();
         hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
         if (e != hipSuccess) return e;
         return lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);

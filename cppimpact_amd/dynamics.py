@@ -308,6 +308,38 @@ class Dynamics:
                 self.ctx)
         return W.value, V.value
 
+    # -- surrogate return map (stress_update=RETURN_MLP) -------------------
+    def set_surrogate(self, mlp):
+        """Installs a copy of a trained Surrogate's weights (before init)."""
+        self._create()
+        L.check(L.lib().cpi_set_surrogate(self.ctx, mlp.handle), self.ctx)
+
+    def surrogate_stats(self):
+        """(points that took the surrogate, points that fell back)."""
+        a, f = C.c_int64(), C.c_int64()
+        L.check(L.lib().cpi_get_surrogate_stats(self.ctx, C.byref(a), C.byref(f)), self.ctx)
+        return a.value, f.value
+
+    def reset_surrogate_stats(self):
+        L.check(L.lib().cpi_reset_surrogate_stats(self.ctx), self.ctx)
+
+    def enable_samples(self, capacity):
+        """Records the exact return map's yielding points (physics EP)."""
+        self._create()
+        L.check(L.lib().cpi_samples_enable(self.ctx, int(capacity)), self.ctx)
+        self._sample_cap = int(capacity)
+
+    def samples(self):
+        """(recorded samples [k][20] float32, number of yielding points seen)."""
+        cap = getattr(self, "_sample_cap", 0)
+        out = np.zeros((max(cap, 1), L.SAMPLE_FLOATS), dtype=np.float32)
+        cnt = C.c_int32()
+        L.check(L.lib().cpi_samples_get(self.ctx, L.fptr(out), cap, C.byref(cnt)), self.ctx)
+        return out[:min(cnt.value, cap)].copy(), cnt.value
+
+    def reset_samples(self):
+        L.check(L.lib().cpi_samples_reset(self.ctx), self.ctx)
+
     def close(self):
         if self.ctx is not None:
             L.lib().cpi_destroy(self.ctx)

@@ -1,0 +1,138 @@
+"""MLP surrogate of the J2 return map (cpi_mlp_* in include/cppimpact_hip.h).
+
+The reference has no learned stress update; its element force is the linear
+K_e du of dynamics_kernels.h:101-114.  This module wraps the network object
+of the HIP library: sample collection from exact-return runs, BF16-MFMA
+training on the GPU, and installation into a Dynamics context created with
+stress_update=RETURN_MLP.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .dynamics import Dynamics
+
+
+class Surrogate:
+    """13 -> 64 -> 64 -> 7 SiLU network for one material."""
+
+    def __init__(self, material, seed=0, device=0):
+        mat = L.Material(material.E, material.rho, material.nu, getattr(material, "beta", 0.0),
+                         getattr(material, "H", 0.0), getattr(material, "Y0", 0.0))
+        h = C.c_void_p()
+        L.check_mlp(L.lib().cpi_mlp_create(int(device), int(seed), C.byref(mat), C.byref(h)))
+        self.handle = h
+        self.material = material
+
+    def _chk(self, rc):
+        return L.check_mlp(rc, self.handle)
+
+    @property
+    def params(self):
+        p = np.zeros(L.MLP_NPARAMS, dtype=np.float32)
+        self._chk(L.lib().cpi_mlp_get_params(self.handle, L.fptr(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (L.MLP_NPARAMS,)
+        self._chk(L.lib().cpi_mlp_set_params(self.handle, L.fptr(p)))
+
+    def scaling(self):
+        a, b = np.zeros(L.MLP_INPUTS, np.float32), np.zeros(L.MLP_INPUTS, np.float32)
+        c, d = np.zeros(L.MLP_OUTPUTS, np.float32), np.zeros(L.MLP_OUTPUTS, np.float32)
+        self._chk(L.lib().cpi_mlp_get_scaling(self.handle, L.fptr(a), L.fptr(b), L.fptr(c),
+                                              L.fptr(d)))
+        return a, b, c, d
+
+    def set_scaling(self, in_mean, in_std, out_mean, out_std):
+        arrs = [np.ascontiguousarray(v, dtype=np.float32) for v in
+                (in_mean, in_std, out_mean, out_std)]
+        self._chk(L.lib().cpi_mlp_set_scaling(self.handle, *(L.fptr(v) for v in arrs)))
+
+    def fit_scaling(self, samples):
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        self._chk(L.lib().cpi_mlp_fit_scaling(self.handle, L.fptr(s), len(s)))
+
+    def targets(self, samples):
+        """Standardized (x, t) of the samples the trainer keeps."""
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        x = np.zeros((max(len(s), 1), L.MLP_INPUTS), np.float32)
+        t = np.zeros((max(len(s), 1), L.MLP_OUTPUTS), np.float32)
+        k = C.c_int32()
+        self._chk(L.lib().cpi_mlp_targets(self.handle, L.fptr(s), len(s), L.fptr(x), L.fptr(t),
+                                          C.byref(k)))
+        return x[:k.value].copy(), t[:k.value].copy()
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), L.MLP_OUTPUTS), np.float32)
+        self._chk(L.lib().cpi_mlp_forward(self.handle, L.fptr(x), len(x), L.fptr(y)))
+        return y
+
+    def gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = np.zeros(L.MLP_NPARAMS, np.float32)
+        loss = C.c_float()
+        self._chk(L.lib().cpi_mlp_gradient(self.handle, L.fptr(x), L.fptr(t), len(x), L.fptr(g),
+                                           C.byref(loss)))
+        return float(loss.value), g
+
+    def train(self, samples, epochs=60, batch=1024, lr=2e-3, lr_final=2e-5, seed=1):
+        o = L.MlpTrainOpts()
+        L.lib().cpi_mlp_default_train_opts(C.byref(o))
+        o.epochs, o.batch, o.lr, o.lr_final, o.seed = epochs, batch, lr, lr_final, seed
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        hist = np.zeros(epochs, np.float32)
+        self._chk(L.lib().cpi_mlp_train(self.handle, L.fptr(s), len(s), C.byref(o),
+                                        L.fptr(hist)))
+        return hist
+
+    def return_map(self, de, sigma_n, ep_n, tolerance=1e-3):
+        """Surrogate return map of the element kernels on given points:
+        (sigma_{n+1}, ep_{n+1}, outcome CPI_MLP_*)."""
+        de = np.ascontiguousarray(de, dtype=np.float32)
+        so = np.ascontiguousarray(sigma_n, dtype=np.float32)
+        ep = np.ascontiguousarray(ep_n, dtype=np.float32)
+        n = len(ep)
+        s = np.zeros((n, 6), np.float32)
+        e = np.zeros(n, np.float32)
+        how = np.zeros(n, np.int32)
+        self._chk(L.lib().cpi_mlp_return_map(self.handle, float(tolerance), n, L.fptr(de),
+                                             L.fptr(so), L.fptr(ep), L.fptr(s), L.fptr(e),
+                                             L.iptr(how)))
+        return s, e, how
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            L.lib().cpi_mlp_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def collect_samples(mesh, material, wall, speeds, dt, nsteps, capacity=1 << 21,
+                    direction=(0.0, 0.0, -1.0), **options):
+    """Runs exact-return impacts (FP32 element kernels unless overridden) at
+    each speed and returns the recorded yielding points [k][20]."""
+    out = []
+    opts = dict(physics=L.PHYSICS_EP, precision=L.PREC_F32)
+    opts.update(options)
+    for v in speeds:
+        dyn = Dynamics(mesh, material, wall, **opts)
+        dyn.vel[:] = np.asarray(direction, dtype=np.float64) * float(v)
+        dyn.enable_samples(capacity)
+        dyn.init(dt)
+        dyn.step(nsteps)
+        dyn.synchronize()
+        s, _ = dyn.samples()
+        out.append(s)
+        dyn.close()
+    return np.concatenate(out) if out else np.zeros((0, L.SAMPLE_FLOATS), np.float32)

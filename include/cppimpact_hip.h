@@ -17,8 +17,9 @@
  * Host arrays use the reference's layouts: positions/velocities/accelerations
  * are [num_nodes][3] doubles (Mesh::xloc, Dynamics::vel), connectivity is
  * [num_elements][nodes_per_element] 0-based int32 (Mesh::element_nodes).
- * The library keeps its own device layout (structure-of-arrays, elements and
- * nodes permuted for locality) and converts on upload/download.
+ * The library keeps its own device layout (one 32-byte record per node per
+ * field, stress state structure-of-arrays, elements and nodes permuted for
+ * locality) and converts on upload/download.
  *
  * Thread safety: one context must not be used from two threads at once.
  */
@@ -247,6 +248,105 @@ int cpi_debug_strain(cpi_ctx *ctx, double alpha, int32_t def_case,
    measures the HBM bandwidth the roofline is compared against and
    calibrates the memory counters (a copy moves a known number of bytes). */
 int cpi_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs);
+
+/* ---- MLP surrogate of the J2 return map --------------------------------- *
+ * No counterpart in the reference (its stress update is the linear
+ * K_e du of dynamics_kernels.h:101-114).  A 13 -> 64 -> 64 -> 7 SiLU network
+ * evaluated with BF16 MFMA (FP32 accumulation) replaces the radial return
+ * at yielding points of CPI_PHYSICS_EP + CPI_RETURN_MLP contexts (tet4,
+ * FP32, SCATTER_BLOCK); a point whose surrogate state misses the yield
+ * surface by more than mlp_tolerance * Y0 takes the exact return.
+ *
+ * Inputs (standardized): strain increment [6], stress_n [6], eq. plastic
+ * strain_n.  Outputs (standardized): (sigma_trial - sigma_{n+1}) / q [6] and
+ * Y0 * dep / q, with q = |dev sigma_trial| - sqrt(2/3) (Y0 + H ep_n) the
+ * trial overshoot.  Parameters, canonical order (5511 floats):
+ * W1[64][13], b1[64], W2[64][64], b2[64], W3[7][64], b3[7].
+ *
+ * A sample, as recorded by cpi_samples_enable, is 20 floats:
+ * de[6], sigma_n[6], ep_n, sigma_{n+1}[6], ep_{n+1}. */
+#define CPI_MLP_INPUTS 13
+#define CPI_MLP_OUTPUTS 7
+#define CPI_MLP_NPARAMS 5511
+#define CPI_SAMPLE_FLOATS 20
+
+/* outcome of one point of cpi_mlp_return_map */
+#define CPI_MLP_ELASTIC 0
+#define CPI_MLP_ACCEPTED 1
+#define CPI_MLP_FALLBACK 2
+
+typedef struct cpi_mlp cpi_mlp;
+
+typedef struct cpi_mlp_train_opts {
+  int32_t epochs;       /* default 60 */
+  int32_t batch;        /* samples per Adam step, default 1024, <= 65536 */
+  float lr, lr_final;   /* cosine decay from lr to lr_final */
+  float beta1, beta2, eps;
+  uint64_t seed;        /* shuffling */
+} cpi_mlp_train_opts;
+
+void cpi_mlp_default_train_opts(cpi_mlp_train_opts *opt);
+
+/* Network tied to a material (D and the yield curve), Glorot-uniform
+   weights from SplitMix64(seed), zero biases, identity scaling. */
+int cpi_mlp_create(int32_t device, uint64_t seed, const cpi_material *mat,
+                   cpi_mlp **out);
+void cpi_mlp_destroy(cpi_mlp *m);
+const char *cpi_mlp_last_error(const cpi_mlp *m);
+
+/* FP32 master parameters, canonical order; setting them clears Adam. */
+int cpi_mlp_set_params(cpi_mlp *m, const float *params);
+int cpi_mlp_get_params(cpi_mlp *m, float *params);
+
+/* standardization: x_std = (x - mean) / std, per feature and output */
+int cpi_mlp_set_scaling(cpi_mlp *m, const float *in_mean, const float *in_std,
+                        const float *out_mean, const float *out_std);
+int cpi_mlp_get_scaling(cpi_mlp *m, float *in_mean, float *in_std,
+                        float *out_mean, float *out_std);
+/* mean / std of the samples with q > 1e-3 Y0 */
+int cpi_mlp_fit_scaling(cpi_mlp *m, const float *samples, int32_t n);
+/* standardized (x[kept][13], t[kept][7]) of the samples with q > 1e-3 Y0 */
+int cpi_mlp_targets(cpi_mlp *m, const float *samples, int32_t n, float *x,
+                    float *t, int32_t *kept);
+
+/* y[n][7] = network(x[n][13]), standardized, through the inference kernel
+   the element pass uses. */
+int cpi_mlp_forward(cpi_mlp *m, const float *x, int32_t n, float *y);
+
+/* Gradient of L = 1/(2n) sum |y - t|^2 over the parameters (canonical
+   order) by the BF16-MFMA training kernel; n <= 65536. */
+int cpi_mlp_gradient(cpi_mlp *m, const float *x, const float *t, int32_t n,
+                     float *grad, float *loss);
+
+/* Adam on samples (scaling fitted first if never set); loss_history gets
+   the mean loss of each epoch (may be NULL). */
+int cpi_mlp_train(cpi_mlp *m, const float *samples, int32_t n,
+                  const cpi_mlp_train_opts *opt, float *loss_history);
+
+/* The element pass's surrogate return map on n points (FP32): trial
+   stress sigma_n + D de, then surrogate or exact return; how[i] is
+   CPI_MLP_ELASTIC / ACCEPTED / FALLBACK. */
+int cpi_mlp_return_map(cpi_mlp *m, double tolerance, int32_t n,
+                       const float *de, const float *sigma_n,
+                       const float *ep_n, float *sigma_out, float *ep_out,
+                       int32_t *how);
+
+/* Installs a copy of m's weights and scaling into a CPI_RETURN_MLP context
+   (the material must match); required before cpi_init. */
+int cpi_set_surrogate(cpi_ctx *ctx, const cpi_mlp *m);
+/* points that took the surrogate / fell back to the exact return since the
+   last reset */
+int cpi_get_surrogate_stats(cpi_ctx *ctx, int64_t *accepted, int64_t *fallback);
+int cpi_reset_surrogate_stats(cpi_ctx *ctx);
+
+/* Records (de, sigma_n, ep_n, sigma_{n+1}, ep_{n+1}) of every yielding
+   point of a CPI_PHYSICS_EP + CPI_RETURN_EXACT context into a device buffer
+   of capacity samples (0 disables); points past the capacity are counted
+   and dropped. */
+int cpi_samples_enable(cpi_ctx *ctx, int32_t capacity);
+/* copies min(recorded, capacity, cap) samples; *recorded = points seen */
+int cpi_samples_get(cpi_ctx *ctx, float *out, int32_t cap, int32_t *recorded);
+int cpi_samples_reset(cpi_ctx *ctx);
 
 /* ---- host-only utilities (no GPU needed) -------------------------------- */
 

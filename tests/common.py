@@ -3,6 +3,7 @@ get the same seeded meshes, materials and walls)."""
 import numpy as np
 
 import cppimpact_amd as ci
+from oracle import mlp_oracle as M
 from oracle import pyoracle as O
 
 E, NU, RHO, SY, H = 68.9e9, 0.33, 2700.0, 276e6, 700e6
@@ -49,3 +50,28 @@ def rel(a, b, scale=None):
     a, b = np.asarray(a), np.asarray(b)
     s = np.abs(b).max() if scale is None else scale
     return float(np.abs(a - b).max() / max(s, 1e-300))
+
+
+def plastic_states(rng, n, sy=SY, h=H):
+    """Random stress states near the yield surface and strain increments
+    that push most of them through it: (de, sigma_n, ep_n)."""
+    ep = rng.uniform(0.0, 0.05, n)
+    d = rng.normal(size=(n, 6))
+    d[:, :3] -= d[:, :3].mean(axis=1, keepdims=True)
+    d /= M.dev_norm(d)[:, None]
+    radius = M.K23 * (sy + h * ep) * rng.uniform(0.9, 1.0, n)
+    so = d * radius[:, None]
+    so[:, :3] += rng.normal(0, 0.3 * sy, n)[:, None]
+    de = rng.normal(0, 2e-4, (n, 6)) + 3e-4 * d
+    return de, so, ep
+
+
+def exact_samples(rng, n):
+    """Samples (de, sigma_n, ep_n, sigma_{n+1}, ep_{n+1}) of the exact
+    return at plastic_states, float32 like the recorder's."""
+    de, so, ep = plastic_states(rng, n)
+    tr = M.trial(de, so, E, NU)
+    s1, ep1, _ = M.radial_return(tr, ep, E, NU, SY, H)
+    keep = M.overshoot(tr, ep, SY, H) > 0
+    smp = np.concatenate([de, so, ep[:, None], s1, ep1[:, None]], axis=1)[keep]
+    return smp.astype(np.float32)
