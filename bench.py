@@ -9,9 +9,11 @@ iteration of Dynamics::solve over the whole mesh, inputs resident in HBM.
 
 Two runs of the same workload: the plain FP64 step (exact radial return) and
 the mixed-precision step (FP32 element kernels, FP64 nodal state, mass, dt
-and energies; MLP surrogate with exact fallback when --surrogate is given and
-weights are installed).  The headline value is the mixed-precision run; the
-plain run, the speedup and the displacement error between them ride along.
+and energies, MLP surrogate of the return map with exact fallback; the
+network is trained first on samples of small-cube impacts at speeds uniform
+in [5, 30] m/s, --no-surrogate keeps the exact return).  The headline value
+is the mixed-precision run; the plain run, the speedup and the displacement
+and plastic-strain errors between them ride along.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -36,7 +38,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--scatter", default="block", choices=["block", "color", "atomic"])
-    ap.add_argument("--surrogate", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="mixed-precision run with the exact return instead of the MLP")
+    ap.add_argument("--speed", type=float, default=10.0, help="impact speed, m/s")
+    ap.add_argument("--train-n", type=int, default=16,
+                    help="hexes per side of the cube the training samples come from")
+    ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_HEX, help="hexes per side (default 64)")
@@ -48,7 +55,35 @@ def make_mesh(n):
     return ci.box_mesh(n, n, n, SIDE, SIDE, SIDE, origin=(0.0, 0.0, 1e-6), jitter=0.1, seed=0)
 
 
-def run_mode(x, conn, args, precision, stress_update, dt):
+def train_surrogate(args):
+    """Samples from exact-return FP32 impacts of a small cube at speeds
+    uniform in [5, 30] m/s, then BF16-MFMA training on the GPU."""
+    import numpy as np
+
+    import cppimpact_amd as ci
+    from cppimpact_amd.surrogate import Surrogate, collect_samples
+
+    E = 68.9e9
+    n = args.train_n
+    x, conn = ci.box_mesh(n, n, n, SIDE, SIDE, SIDE, origin=(0.0, 0.0, 1e-6), jitter=0.1, seed=1)
+    mat = ci.Elastoplastic(E, 2700.0, 0.33, 1.0, 700e6, 276e6)
+    wall = ci.Wall("Wall", 0.0, 10 * E * SIDE / n, None, 0, 1)
+    speeds = np.random.default_rng(0).uniform(5.0, 30.0, 6)
+    t0 = time.time()
+    smp = collect_samples(ci.Mesh.from_arrays(x, conn), mat, wall, speeds, 0.0, 400,
+                          capacity=1 << 20)
+    t1 = time.time()
+    net = Surrogate(mat, seed=0)
+    hist = net.train(smp, epochs=args.train_epochs, batch=1024)
+    t2 = time.time()
+    info = {"samples": int(len(smp)), "speeds": [round(float(v), 2) for v in speeds],
+            "mesh": f"cube {n}^3 x 6 tet4, 400 steps per speed", "epochs": args.train_epochs,
+            "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
+            "collect_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2)}
+    return net, info
+
+
+def run_mode(x, conn, args, precision, stress_update, dt, net=None):
     import numpy as np
 
     import cppimpact_amd as ci
@@ -61,7 +96,9 @@ def run_mode(x, conn, args, precision, stress_update, dt):
     scatter = {"block": L.SCATTER_BLOCK, "color": L.SCATTER_COLOR, "atomic": L.SCATTER_ATOMIC}
     dyn = ci.Dynamics(mesh, mat, wall, physics=L.PHYSICS_EP, precision=precision,
                       stress_update=stress_update, scatter=scatter[args.scatter])
-    dyn.vel[:] = (0.0, 0.0, -10.0)
+    dyn.vel[:] = (0.0, 0.0, -args.speed)
+    if net is not None:
+        dyn.set_surrogate(net)
     t0 = time.time()
     dyn.init(dt)
     setup = time.time() - t0
@@ -77,6 +114,7 @@ def run_mode(x, conn, args, precision, stress_update, dt):
     xg, *_ = dyn.fetch()
     energy, step = dyn.energy()
     _, ep = dyn.element_state()
+    stats = dyn.surrogate_stats() if net is not None else None
     dyn.close()
     ne = info.num_elements
     elem_name = [k for k in prof if k.startswith("elem")][0]
@@ -91,7 +129,8 @@ def run_mode(x, conn, args, precision, stress_update, dt):
         elem_bytes=info.elem_kernel_bytes, node_bytes=info.node_kernel_bytes,
         step_bytes=info.bytes_per_step, x=xg, energy=energy.tolist(), step=step,
         epbar_max=float(np.max(ep)), blocks=info.num_blocks,
-        boundary_nodes=info.num_boundary_nodes, colors=info.num_colors,
+        boundary_nodes=info.num_boundary_nodes, colors=info.num_colors, ep=ep,
+        surrogate_points=stats,
     )
 
 
@@ -162,13 +201,21 @@ def main():
     plain["elem_kernel_key"] = "f64"
     print(f"bench: fp64 {plain['value']:.4g} el-steps/s, elem kernel {plain['elem_ms']:.4f} ms",
           file=sys.stderr, flush=True)
-    ret = L.RETURN_MLP if args.surrogate else L.RETURN_EXACT
-    mixed = run_mode(x, conn, args, L.PREC_F32, ret, plain["dt"])
-    mixed["elem_kernel_key"] = "f32_mlp" if args.surrogate else "f32"
+    net, train = None, None
+    if not args.no_surrogate:
+        net, train = train_surrogate(args)
+        print(f"bench: surrogate trained on {train['samples']} samples in "
+              f"{train['train_s']} s (loss {train['loss_first']:.3g} -> {train['loss_last']:.3g})",
+              file=sys.stderr, flush=True)
+    ret = L.RETURN_EXACT if args.no_surrogate else L.RETURN_MLP
+    mixed = run_mode(x, conn, args, L.PREC_F32, ret, plain["dt"], net)
+    mixed["elem_kernel_key"] = "f32" if args.no_surrogate else "f32_mlp"
     print(f"bench: fp32 {mixed['value']:.4g} el-steps/s, elem kernel {mixed['elem_ms']:.4f} ms",
           file=sys.stderr, flush=True)
     disp = np.abs(plain["x"] - x).max()
     derr = float(np.abs(mixed["x"] - plain["x"]).max() / disp) if disp > 0 else 0.0
+    epmax = float(np.abs(plain["ep"]).max())
+    eperr = float(np.abs(mixed["ep"] - plain["ep"]).max() / epmax) if epmax > 0 else 0.0
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, conn, plain["dt"], args.cpu_steps)
@@ -188,9 +235,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"cube {args.n}^3 hexes x 6 tet4 ({conn.shape[0]} elements, "
-                        f"{x.shape[0]} nodes), jitter 0.1 seed 0, J2 ep, v0 -10 m/s, "
-                        f"wall k=10*E*h, dt=0.5*stable",
-            "mode": "fp32 element kernels" + (" + MLP surrogate" if args.surrogate else "")
+                        f"{x.shape[0]} nodes), jitter 0.1 seed 0, J2 ep, v0 -{args.speed:g} "
+                        f"m/s, wall k=10*E*h, dt=0.5*stable",
+            "mode": "fp32 element kernels" + ("" if args.no_surrogate else " + MLP surrogate")
                     + ", fp64 nodal state/mass/dt/energy",
             "scatter": args.scatter,
             "dt": plain["dt"],
@@ -202,12 +249,18 @@ def main():
                        "roofline": roofline(plain)},
         "speedup_vs_fp64": round(mixed["value"] / plain["value"], 4),
         "disp_rel_err_vs_fp64": derr,
+        "epbar_rel_err_vs_fp64": eperr,
+        "surrogate": None if net is None else dict(
+            train, points_accepted=mixed["surrogate_points"][0],
+            points_fell_back=mixed["surrogate_points"][1]),
         "gpu_event_ms_per_step": round(mixed["gpu_ms_per_step"], 5),
         "hbm_stream_copy_gbs": round(gbs.value, 1),
         "node_kernel_ms": round(mixed["node_ms"], 5),
         "blocks": mixed["blocks"], "boundary_nodes": mixed["boundary_nodes"],
         "epbar_max": mixed["epbar_max"],
     }
+    if net is not None:
+        net.close()
     print(json.dumps(line), flush=True)
 
 

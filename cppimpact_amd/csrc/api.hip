@@ -86,7 +86,10 @@ int dalloc(cpi_ctx *c, T **p, size_t n) {
 template <typename T>
 int upload(cpi_ctx *c, T *d, const std::vector<T> &h) {
   if (h.empty()) return CPI_OK;
-  HIPCHK(c, hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  // on the context's stream: kernels there do not wait for the null stream,
+  // and a pageable host-to-device copy may return before its DMA lands
+  HIPCHK(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return CPI_OK;
 }
 
@@ -558,7 +561,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   P.hist_cap = 4096;
   P.mlp_tol = opt->mlp_tolerance;
   if (c->phys == 2) {
-    if ((rc = mlp_alloc(c->mlp, c->err))) return rc;
+    if ((rc = mlp_alloc(c->mlp, c->stream, c->err))) return rc;
     P.mlp = c->mlp.d_params;
     P.mlp_stats = c->mlp.d_stats;
   }
@@ -708,7 +711,8 @@ extern "C" int cpi_get_state(cpi_ctx *c, double *x, double *v, double *a, double
     for (long i = 0; i < 3L * c->nn; i++) vi[i] = hv[i] - 0.5 * c->dt * ha[i];
   if (m) {
     std::vector<double> hm(c->nn);
-    HIPCHK(c, hipMemcpy(hm.data(), c->P.m, sizeof(double) * c->nn, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(hm.data(), c->P.m, sizeof(double) * c->nn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < c->nn; i++) m[c->nperm[i]] = hm[i];
   }
   return c->inited ? check_flags(c) : CPI_OK;
@@ -727,10 +731,12 @@ extern "C" int cpi_get_element_state(cpi_ctx *c, double *sig, double *ep) {
     out.resize(cnt);
     if (c->prec == CPI_PREC_F32) {
       std::vector<float> t(cnt);
-      HIPCHK(c, hipMemcpy(t.data(), d, 4 * cnt, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpyAsync(t.data(), d, 4 * cnt, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
       for (size_t i = 0; i < cnt; i++) out[i] = t[i];
     } else {
-      HIPCHK(c, hipMemcpy(out.data(), d, 8 * cnt, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpyAsync(out.data(), d, 8 * cnt, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return CPI_OK;
   };
@@ -793,7 +799,8 @@ extern "C" int cpi_get_energy_history(cpi_ctx *c, double *rows, int32_t cap, int
   const int64_t have = std::min<int64_t>(h.nrows, c->P.hist_cap);
   const int64_t n = std::min<int64_t>(have, cap);
   std::vector<double> all(8 * (size_t)c->P.hist_cap);
-  HIPCHK(c, hipMemcpy(all.data(), c->P.hist, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpyAsync(all.data(), c->P.hist, sizeof(double) * all.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int64_t i = 0; i < n; i++) {
     int64_t r = h.nrows - n + i;
     std::memcpy(rows + 8 * i, &all[8 * (size_t)(r % c->P.hist_cap)], 8 * sizeof(double));
@@ -1067,7 +1074,7 @@ extern "C" int cpi_samples_enable(cpi_ctx *c, int32_t cap) {
     return fail(c, CPI_ERR_STATE, "cpi_samples_enable: needs CPI_PHYSICS_EP with the exact return");
   HIPCHK(c, hipSetDevice(c->dev));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  int rc = mlp_samples_alloc(c->mlp, cap, c->err);
+  int rc = mlp_samples_alloc(c->mlp, cap, c->stream, c->err);
   if (rc) return rc;
   c->P.samples = c->mlp.d_samples;
   c->P.sample_count = c->mlp.d_sample_count;

@@ -225,20 +225,21 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
 // tet10: loop over the rule's points with coordinates relative to corner 2
 // (the origin of the reference's quadratic parametrisation); nodes are
 // re-read through get() at every pass instead of being held in registers.
+// Corner forces accumulate through add(i, dN_i, sigma, w detJ), which adds
+// w detJ sigma . dN_i to node i's force, one point at a time in rule order.
 // ---------------------------------------------------------------------------
-template <int RULE, typename R, int PHYS, class Get, class Put>
-__device__ __forceinline__ void tet10_core(const Params &p, const Get &get, double dt, int e,
-                                           const Put &put, EAcc &ea) {
+template <int RULE, typename R, int PHYS, class Get, class Add>
+__device__ __forceinline__ void tet10_points(const Params &p, const Get &get, double dt, int e,
+                                             const Add &add, EAcc &ea) {
   double o[3];
   {
     double ov[3];
     get(2, o, ov);
   }
-  R f[10][3];
-#pragma unroll
-  for (int i = 0; i < 10; i++) f[i][0] = f[i][1] = f[i][2] = R(0);
   using Q = Quad<RULE>;
-#pragma unroll
+  // a rolled loop: unrolled, the compiler hoists every point's state loads
+  // and the kernel no longer fits the register file
+#pragma unroll 1
   for (int q = 0; q < Q::n; q++) {
     double Gd[30];
     tet10_grad(Q::pt(q, 0), Q::pt(q, 1), Q::pt(q, 2), Gd);
@@ -285,7 +286,7 @@ __device__ __forceinline__ void tet10_core(const Params &p, const Get &get, doub
       for (int c = 0; c < 3; c++)
         dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
                 Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
-      add_bts<R>(f[i], dn, s, wd);
+      add(i, dn, s, wd);
     }
   }
   if constexpr (PHYS != PHYS_COMPAT) {
@@ -305,6 +306,17 @@ __device__ __forceinline__ void tet10_core(const Params &p, const Get &get, doub
            e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
     ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, e3, v6, p.wave_speed, kTet10LengthFactor));
   }
+}
+
+// tet10 with the corner forces summed in registers, handed to put() at the end
+template <int RULE, typename R, int PHYS, class Get, class Put>
+__device__ __forceinline__ void tet10_core(const Params &p, const Get &get, double dt, int e,
+                                           const Put &put, EAcc &ea) {
+  R f[10][3];
+#pragma unroll
+  for (int i = 0; i < 10; i++) f[i][0] = f[i][1] = f[i][2] = R(0);
+  auto add = [&](int i, const R dn[3], const R s[6], R wd) { add_bts<R>(f[i], dn, s, wd); };
+  tet10_points<RULE, R, PHYS>(p, get, dt, e, add, ea);
 #pragma unroll
   for (int i = 0; i < 10; i++) put(i, f[i]);
 }

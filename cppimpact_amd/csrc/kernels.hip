@@ -138,9 +138,8 @@ __device__ __forceinline__ void stage_node(double *d, const Node4 &xm, const Nod
 }
 
 // ---------------------------------------------------------------------------
-// LDS-assembled element pass, generic (tet10): loops over elements and
-// nodes.  LDS: node records [max_loc][8] doubles, corner forces
-// [max_el*NPE][3] R.
+// LDS-assembled element pass, tet10: loops over elements and nodes.  LDS:
+// node records [max_loc][8] doubles, corner forces [max_el*NPE][3] R.
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, typename R, int PHYS, bool AUX, int NT>
 __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
@@ -174,12 +173,22 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
         vk[c] = d[4 + c];
       }
     };
-    auto put = [&](int k, const R fk[3]) {
-      R *dst = lf + 3 * pos[k];
+    // the element's corner slots belong to this thread alone: its forces
+    // accumulate there point by point instead of in registers
 #pragma unroll
-      for (int c = 0; c < 3; c++) dst[c] = fk[c];
+    for (int k = 0; k < NPE; k++) {
+      R *dst = lf + 3 * pos[k];
+      dst[0] = dst[1] = dst[2] = R(0);
+    }
+    auto add = [&](int k, const R dn[3], const R s[6], R wd) {
+      R *dst = lf + 3 * pos[k];
+      R fk[3] = {dst[0], dst[1], dst[2]};
+      add_bts<R>(fk, dn, s, wd);
+      dst[0] = fk[0];
+      dst[1] = fk[1];
+      dst[2] = fk[2];
     };
-    elem_core<NPE, RULE, R, PHYS>(p, get, dt, e, put, ea);
+    tet10_points<RULE, R, PHYS>(p, get, dt, e, add, ea);
   }
   __syncthreads();
 

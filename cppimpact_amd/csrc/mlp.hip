@@ -341,22 +341,23 @@ __global__ void __launch_bounds__(64) k_mlp_return_points(Params p, int n, const
 // per-context buffers
 // ---------------------------------------------------------------------------
 
-int mlp_alloc(MlpHost &m, std::string &err) {
+int mlp_alloc(MlpHost &m, hipStream_t s, std::string &err) {
   if (hipMalloc(&m.d_params, sizeof(MlpParams)) != hipSuccess ||
       hipMalloc(&m.d_weights, sizeof(MlpWeights)) != hipSuccess ||
       hipMalloc(&m.d_stats, 2 * sizeof(unsigned long long)) != hipSuccess) {
     err = "hipMalloc for the surrogate failed";
     return CPI_ERR_ALLOC;
   }
-  if (hipMemset(m.d_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess ||
-      hipMemset(m.d_weights, 0, sizeof(MlpWeights)) != hipSuccess) {
+  if (hipMemsetAsync(m.d_stats, 0, 2 * sizeof(unsigned long long), s) != hipSuccess ||
+      hipMemsetAsync(m.d_weights, 0, sizeof(MlpWeights), s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
     err = "hipMemset for the surrogate failed";
     return CPI_ERR_HIP;
   }
   return CPI_OK;
 }
 
-int mlp_samples_alloc(MlpHost &m, int cap, std::string &err) {
+int mlp_samples_alloc(MlpHost &m, int cap, hipStream_t s, std::string &err) {
   if (m.d_samples) (void)hipFree(m.d_samples);
   m.d_samples = nullptr;
   m.sample_cap = 0;
@@ -364,7 +365,8 @@ int mlp_samples_alloc(MlpHost &m, int cap, std::string &err) {
     err = "hipMalloc for the sample counter failed";
     return CPI_ERR_ALLOC;
   }
-  if (hipMemset(m.d_sample_count, 0, sizeof(int)) != hipSuccess) {
+  if (hipMemsetAsync(m.d_sample_count, 0, sizeof(int), s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
     err = "hipMemset of the sample counter failed";
     return CPI_ERR_HIP;
   }
@@ -668,8 +670,9 @@ extern "C" int cpi_mlp_create(int32_t device, uint64_t seed, const cpi_material 
       (rc = mdalloc(m, &m->lpart, kMaxParts)) || (rc = mdalloc(m, &m->gsum, kMlpNParams + 1)))
     return bail(rc);
   m->part_cap = kMaxParts;
-  if (hipMemset(m->W, 0, sizeof(MlpWeights)) != hipSuccess ||
-      hipMemset(m->TW, 0, sizeof(MlpTrainWeights)) != hipSuccess)
+  if (hipMemsetAsync(m->W, 0, sizeof(MlpWeights), m->stream) != hipSuccess ||
+      hipMemsetAsync(m->TW, 0, sizeof(MlpTrainWeights), m->stream) != hipSuccess ||
+      hipStreamSynchronize(m->stream) != hipSuccess)
     return bail(mfail(m, CPI_ERR_HIP, "hipMemset failed"));
   // Glorot-uniform weights, zero biases
   std::vector<float> theta(kMlpNParams, 0.f);

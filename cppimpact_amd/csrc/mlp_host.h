@@ -27,9 +27,9 @@ struct MlpHost {
 void set_global_error(const std::string &s);
 const char *global_error();
 
-int mlp_alloc(MlpHost &m, std::string &err);
+int mlp_alloc(MlpHost &m, hipStream_t s, std::string &err);
 void mlp_free(MlpHost &m);
-int mlp_samples_alloc(MlpHost &m, int cap, std::string &err);
+int mlp_samples_alloc(MlpHost &m, int cap, hipStream_t s, std::string &err);
 
 // material constants the network is tied to (D and the yield curve)
 struct MlpMaterial {

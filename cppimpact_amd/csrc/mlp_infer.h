@@ -19,70 +19,77 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // All 64 lanes of the wave must call this together.  x: the lane's 13
 // standardized inputs; y: its 7 standardized outputs.  stage: this wave's
-// kMlpStageBytes of LDS.
+// kMlpStageBytes of LDS.  Each hidden tile is packed to BF16 as soon as it
+// is produced, and the output layer accumulates per K-step, so few hidden
+// values are live at once.
 __device__ __forceinline__ void mlp_wave_infer(const MlpParams &P, const float x[kMlpIn],
                                                float y[kMlpOut], uint16_t *stage) {
   const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
   // inputs as bf16 rows [64 samples][16 features]
 #pragma unroll
-  for (int k = 0; k < 16; k++) stage[l * 16 + k] = k < kMlpIn ? f2bf(x[k]) : (uint16_t)0;
+  for (int k = 0; k < 16; k += 2) {
+    const uint32_t lo = f2bf(x[k]), hi = k + 1 < kMlpIn ? f2bf(x[k + 1]) : 0u;
+    *(uint32_t *)(stage + l * 16 + k) = k < kMlpIn ? lo | (hi << 16) : 0u;
+  }
   wave_lds_sync();
-  // layer-1 B operands of the four sample tiles (features 8g..8g+7; g >= 2
-  // is the zero padding of K = 32)
-  bf16x8 xb[4];
   const float zf[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   const bf16x8 zero8 = pack_bf8(zf);
-#pragma unroll
-  for (int st = 0; st < 4; st++) xb[st] = g < 2 ? ld_bf8(stage + (16 * st + n) * 16 + 8 * g) : zero8;
-  wave_lds_sync();
-  float *out = (float *)stage;  // [64][8] floats, written after the inputs are consumed
+  float out[4][4];
 #pragma unroll 1
   for (int st = 0; st < 4; st++) {
-    // layer 1: Z1^T tile mt = W1[16mt..][0..31] x X^T
-    float h[4][4];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = mfma16(ld_bf8(P.w1 + (16 * mt + n) * 32 + 8 * g), xb[st], acc);
-#pragma unroll
-      for (int j = 0; j < 4; j++) h[mt][j] = silu(acc[j] + P.b1[16 * mt + 4 * g + j]);
-    }
-    // layer 2
+    // keeps the weight loads inside the loop (hoisted, they would hold ~90
+    // registers for the whole element kernel)
+    __asm__ volatile("" ::: "memory");
+    // layer-1 B operand: features 8g..8g+7 of sample 16st+n (g >= 2: the
+    // zero padding of K = 32)
+    const bf16x8 xb = g < 2 ? ld_bf8(stage + (16 * st + n) * 16 + 8 * g) : zero8;
     bf16x8 hb[2];
 #pragma unroll
     for (int kk = 0; kk < 2; kk++) {
-      const float v[8] = {h[2 * kk][0], h[2 * kk][1], h[2 * kk][2], h[2 * kk][3],
-                          h[2 * kk + 1][0], h[2 * kk + 1][1], h[2 * kk + 1][2], h[2 * kk + 1][3]};
-      hb[kk] = pack_bf8(v);
-    }
-    float h2[4][4];
+      f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
+      a0 = mfma16(ld_bf8(P.w1 + (32 * kk + n) * 32 + 8 * g), xb, a0);
+      a1 = mfma16(ld_bf8(P.w1 + (32 * kk + 16 + n) * 32 + 8 * g), xb, a1);
+      float v[8];
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < 2; kk++)
-        acc = mfma16(ld_bf8(P.w2p + (16 * mt + n) * 64 + 32 * kk + 8 * g), hb[kk], acc);
-#pragma unroll
-      for (int j = 0; j < 4; j++) h2[mt][j] = silu(acc[j] + P.b2[16 * mt + 4 * g + j]);
-    }
-    // layer 3 (16 output rows, 7 used)
-#pragma unroll
-    for (int kk = 0; kk < 2; kk++) {
-      const float v[8] = {h2[2 * kk][0], h2[2 * kk][1], h2[2 * kk][2], h2[2 * kk][3],
-                          h2[2 * kk + 1][0], h2[2 * kk + 1][1], h2[2 * kk + 1][2],
-                          h2[2 * kk + 1][3]};
+      for (int j = 0; j < 4; j++) {
+        v[j] = silu(a0[j] + P.b1[32 * kk + 4 * g + j]);
+        v[4 + j] = silu(a1[j] + P.b1[32 * kk + 16 + 4 * g + j]);
+      }
       hb[kk] = pack_bf8(v);
     }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kk = 0; kk < 2; kk++) acc = mfma16(ld_bf8(P.w3p + n * 64 + 32 * kk + 8 * g), hb[kk], acc);
-    if (g < 2)
+    for (int kk = 0; kk < 2; kk++) {
+      f32x4 a0 = {0.f, 0.f, 0.f, 0.f}, a1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int j = 0; j < 4; j++) out[(16 * st + n) * 8 + 4 * g + j] = acc[j] + P.b3[4 * g + j];
+      for (int k2 = 0; k2 < 2; k2++) {
+        a0 = mfma16(ld_bf8(P.w2p + (32 * kk + n) * 64 + 32 * k2 + 8 * g), hb[k2], a0);
+        a1 = mfma16(ld_bf8(P.w2p + (32 * kk + 16 + n) * 64 + 32 * k2 + 8 * g), hb[k2], a1);
+      }
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v[j] = silu(a0[j] + P.b2[32 * kk + 4 * g + j]);
+        v[4 + j] = silu(a1[j] + P.b2[32 * kk + 16 + 4 * g + j]);
+      }
+      // output layer (16 rows, 7 used), K-step kk
+      acc = mfma16(ld_bf8(P.w3p + n * 64 + 32 * kk + 8 * g), pack_bf8(v), acc);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[st][j] = acc[j] + P.b3[4 * g + j];
   }
   wave_lds_sync();
+  // lane n+16g holds outputs 4g..4g+3 of samples 16st+n: through LDS to the
+  // sample's own lane
+  float *o = (float *)stage;  // [64][8]
+  if (g < 2)
 #pragma unroll
-  for (int o = 0; o < kMlpOut; o++) y[o] = out[l * 8 + o];
+    for (int st = 0; st < 4; st++)
+      *(float4 *)(o + (16 * st + n) * 8 + 4 * g) =
+          make_float4(out[st][0], out[st][1], out[st][2], out[st][3]);
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < kMlpOut; k++) y[k] = o[l * 8 + k];
   wave_lds_sync();
 }
 

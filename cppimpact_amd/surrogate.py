@@ -135,4 +135,9 @@ def collect_samples(mesh, material, wall, speeds, dt, nsteps, capacity=1 << 21,
         s, _ = dyn.samples()
         out.append(s)
         dyn.close()
-    return np.concatenate(out) if out else np.zeros((0, L.SAMPLE_FLOATS), np.float32)
+    if not out:
+        return np.zeros((0, L.SAMPLE_FLOATS), np.float32)
+    smp = np.concatenate(out)
+    # the recorder fills slots in atomic-counter order; sorting the rows makes
+    # the set (and so the training run) independent of that order
+    return smp[np.lexsort(smp.T[::-1])]

@@ -143,8 +143,11 @@ def _impact(x, conn, v, **opts):
 
 
 def test_surrogate_step_against_exact_return():
-    """Ablation: the same impact with the exact return, with the surrogate
-    (tolerance 1e-3) and with the surrogate forced to fall back everywhere."""
+    """Ablation on one plastic impact: the exact return; the trained
+    surrogate (tolerance 1e-3); an untrained network of the same shape (what
+    the method does without the learned part: the residual check sends its
+    points back to the exact return); the trained surrogate forced to fall
+    back everywhere (tolerance 0), which must reproduce the exact run."""
     x, conn = cube(6, gap=1e-6)
     dt, steps = 0.0, 150  # dt 0: half the stable step, the same in every run
     smp = collect_samples(ci.Mesh.from_arrays(x, conn), material(),
@@ -152,7 +155,10 @@ def test_surrogate_step_against_exact_return():
                           dt, steps)
     assert len(smp) > 2000
     net = Surrogate(material(), seed=7)
-    net.train(smp, epochs=30, batch=512)
+    hist = net.train(smp, epochs=80, batch=512)
+    raw = Surrogate(material(), seed=7)
+    raw.fit_scaling(smp)
+    print(f"samples {len(smp)} loss {hist[0]:.3e} -> {hist[-1]:.3e}")
 
     ref = _impact(x, conn, 30.0)
     ref.init(dt)
@@ -162,26 +168,35 @@ def test_surrogate_step_against_exact_return():
     assert ep_r.max() > 0  # the impact yields
 
     out = {}
-    for tol in (1e-3, 0.0):
+    for name, m, tol in (("trained", net, 1e-3), ("untrained", raw, 1e-3), ("forced", net, 0.0)):
         d = _impact(x, conn, 30.0, stress_update=L.RETURN_MLP, mlp_tolerance=tol)
-        d.set_surrogate(net)
+        d.set_surrogate(m)
         d.init(dt)
         d.step(steps)
-        out[tol] = (d.fetch()[0], *d.element_state(), d.surrogate_stats(), d.energy()[0])
+        xd = d.fetch()[0]
+        sd, ed = d.element_state()
+        acc, fb = d.surrogate_stats()
+        out[name] = dict(disp=rel(xd - x, xr - x), stress=rel(sd, sig_r), ep=rel(ed, ep_r),
+                         accepted=acc, fallback=fb, energy_ok=bool(np.isfinite(d.energy()[0]).all()))
         d.close()
-    # forced fallback reproduces the exact return up to FP32 rounding (the
-    # two runs use different kernel instantiations; plastic flow amplifies
-    # last-bit differences over the steps)
-    xf, sf, ef, (af, ff), _ = out[0.0]
-    xs, ss, es, (a1, f1), en = out[1e-3]
-    print(f"fallback-only: disp {rel(xf - x, xr - x):.3e} stress {rel(sf, sig_r):.3e}; "
-          f"surrogate: accepted {a1} fallback {f1} disp {rel(xs - x, xr - x):.3e} "
-          f"stress {rel(ss, sig_r):.3e} ep {rel(es, ep_r):.3e}")
-    assert af == 0 and ff > 0
-    assert rel(xf - x, xr - x) < 2e-4 and rel(sf, sig_r) < 2e-3
-    # the surrogate run stays near the exact run
-    assert a1 + f1 > 0
-    assert rel(xs - x, xr - x) < 2e-2
-    assert np.isfinite(en).all()
+        print(name, out[name])
+    # forced fallback reproduces the exact return (same arithmetic; a bound
+    # at FP32 rounding in case the two kernel builds contract differently)
+    f = out["forced"]
+    assert f["accepted"] == 0 and f["fallback"] > 0
+    assert f["disp"] < 2e-4 and f["stress"] < 2e-3 and f["ep"] < 2e-3
+    # the residual check rejects much of the untrained network's output (it
+    # accepts points whose overshoot q is itself below the tolerance, where
+    # any bounded correction is within it) ...
+    u = out["untrained"]
+    assert u["fallback"] >= 0.3 * (u["accepted"] + u["fallback"])
+    # ... the trained one replaces the return map at nearly every yielding
+    # point, stays near the exact run, and is several times closer to it
+    t = out["trained"]
+    assert t["accepted"] >= 0.9 * (t["accepted"] + t["fallback"]) > 0
+    assert t["disp"] < 1e-2 and t["ep"] < 2e-2 and t["stress"] < 6e-2 and t["energy_ok"]
+    for k in ("disp", "stress", "ep"):
+        assert 3 * t[k] < u[k]
     ref.close()
     net.close()
+    raw.close()
