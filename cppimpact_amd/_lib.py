@@ -112,6 +112,7 @@ SIGNATURES = {
     "cpi_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "cpi_synchronize": (C.c_int, [C.c_void_p]),
     "cpi_get_state": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "cpi_get_increment": (C.c_int, [C.c_void_p, _dp]),
     "cpi_get_element_state": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cpi_get_stress_strain": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cpi_get_energy": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int64)]),

@@ -718,6 +718,12 @@ extern "C" int cpi_get_state(cpi_ctx *c, double *x, double *v, double *a, double
   return c->inited ? check_flags(c) : CPI_OK;
 }
 
+extern "C" int cpi_get_increment(cpi_ctx *c, double *du) {
+  if (!c || !du) return CPI_ERR_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return get_nodes(c, c->P.du, du);
+}
+
 extern "C" int cpi_get_element_state(cpi_ctx *c, double *sig, double *ep) {
   if (!c) return CPI_ERR_INVALID;
   if (c->phys == CPI_PHYSICS_COMPAT) {

@@ -204,6 +204,10 @@ int cpi_synchronize(cpi_ctx *ctx);
 int cpi_get_state(cpi_ctx *ctx, double *x, double *v, double *a, double *m,
                   double *v_i);
 
+/* du of the last completed step [num_nodes][3] (Dynamics::global_dof,
+   dynamics.h:473-476); zeros before the first step. */
+int cpi_get_increment(cpi_ctx *ctx, double *du);
+
 /* Quadrature-point state: sig [num_elements][nqp][6] (xx yy zz xy yz xz),
    epbar [num_elements][nqp].  Either may be NULL. */
 int cpi_get_element_state(cpi_ctx *ctx, double *sig, double *epbar);

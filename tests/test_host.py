@@ -220,3 +220,15 @@ def test_invalid_inputs_are_rejected_before_the_device():
     nn, ne = C.c_int32(), C.c_int32()
     assert lib.cpi_mesh_box(C.byref(s), C.byref(nn), C.byref(ne), None, None) == -1
     assert b"invalid" in lib.cpi_last_error(None)
+
+
+def test_cpp_driver_reports_missing_mesh():
+    """Mesh::load_mesh returns 1 for a file it cannot open (mesh.h:57-70);
+    the driver stops before any GPU call, as test.cu:67-71 does."""
+    import subprocess
+    exe = os.path.join(ROOT, "cppimpact_amd", "examples", "impact_linear")
+    if not os.path.exists(exe):
+        pytest.skip("example drivers not built")
+    p = subprocess.run([exe, "--mesh", "/nonexistent/mesh.inp"], capture_output=True, text=True,
+                       timeout=60)
+    assert p.returncode == 1 and "Error loading mesh" in p.stderr
