@@ -401,3 +401,39 @@ def test_ep_energy_balance_converges_second_order():
     assert 3.0 < e1 / e2 < 5.0
     # with the penalty wall engaged the error stays small at half the step
     assert _balance(0.5, T, True) < 2e-2
+
+
+def test_stress_strain_of_a_linear_field_is_its_closed_form():
+    """u = A x gives strain (A11, A22, A33, A12+A21, A23+A32, A13+A31) at any
+    point and stress = D strain (engineering shear, Voigt xx,yy,zz,xy,yz,xz)."""
+    rng = np.random.default_rng(7)
+    A = rng.normal(scale=1e-3, size=(3, 3))
+    want = np.array([A[0, 0], A[1, 1], A[2, 2], A[0, 1] + A[1, 0], A[1, 2] + A[2, 1],
+                     A[0, 2] + A[2, 0]])
+    D = np.zeros((6, 6))
+    D[:3, :3] = LAM
+    D[:3, :3] += 2 * MU * np.eye(3)
+    D[3:, 3:] = MU * np.eye(3)
+    for npe, nodes in ((4, NODES4), (10, NODES10)):
+        xe = nodes @ np.array([[1.0, 0.2, 0.1], [0.0, 0.9, 0.3], [0.1, 0.0, 1.1]]).T
+        ue = xe @ A.T
+        for pt in ([0.25, 0.25, 0.25], [0.1, 0.2, 0.3]):
+            st, ss = O.stress_strain(npe, E, NU, xe, ue, pt)
+            assert np.allclose(st, want, rtol=1e-10, atol=1e-16)
+            assert np.allclose(ss, D @ want, rtol=1e-9, atol=1e-3)
+
+
+def test_ep_element_force_in_the_elastic_range_equals_ke_du():
+    """From a stress-free state and an increment below yield, the EP force
+    sum_q w detJ B^T (sigma_n + D B du) equals K_e du on the same rule."""
+    rng = np.random.default_rng(8)
+    for npe, nodes, rule in ((4, NODES4, O.Q_EXACT4), (4, NODES4, O.Q_CENTROID),
+                             (10, NODES10, O.Q_EXACT4)):
+        xe = nodes * 0.01 + rng.uniform(-5e-4, 5e-4, nodes.shape)
+        du = rng.normal(scale=1e-9, size=3 * npe)
+        nq = 1 if rule == O.Q_CENTROID else 4
+        f, s, ep, _ = O.element_force_ep(npe, rule, E, NU, 276e6, 700e6, xe, du,
+                                         np.zeros(6 * nq), np.zeros(nq))
+        fk = O.element_force_compat(npe, rule, E, NU, xe, du)
+        assert np.abs(ep).max() == 0.0
+        assert np.allclose(f, fk, rtol=1e-9, atol=1e-12 * np.abs(fk).max())
