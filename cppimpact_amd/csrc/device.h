@@ -150,6 +150,18 @@ struct EAcc {
   double ke = 0.0, pen = 0.0, fc = 0.0, wgrav = 0.0;
 };
 
+// yield function of a trial stress: |dev s| - sqrt(2/3) (sy0 + H ep), the
+// same arithmetic as radial_return's test
+template <typename R>
+__device__ __forceinline__ R yield_overshoot(const Params &p, const R s[6], R ep) {
+  const R third = R(1) / R(3);
+  R pm = (s[0] + s[1] + s[2]) * third;
+  R d0 = s[0] - pm, d1 = s[1] - pm, d2 = s[2] - pm;
+  R nrm = sqrt(d0 * d0 + d1 * d1 + d2 * d2 + R(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+  const R k23 = R(0.81649658092772603273);  // sqrt(2/3)
+  return nrm - k23 * (R(p.sy0) + R(p.H) * ep);
+}
+
 // J2 return map, linear isotropic hardening (Simo & Hughes 1998 Box 3.2
 // restricted to isotropic hardening); s holds the trial stress on entry.
 template <typename R>

@@ -75,13 +75,12 @@ template <typename R> struct PreState {
 // Stress at one point from the strain increment de.  compat: sigma = D de
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
-// point's internal work and plastic dissipation (times wd) to ea.  An
-// inactive point (a surrogate lane with no element) stores nothing.
+// point's internal work and plastic dissipation (times wd) to ea.  (The
+// surrogate's return map runs in its own workgroup phase, k_elem_block4_mlp.)
 template <typename R, int PHYS, bool PRE = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
-                                             const PreState<R> *pre = nullptr,
-                                             bool active = true) {
+                                             const PreState<R> *pre = nullptr) {
   const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
   R ds[6];
   ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
@@ -115,21 +114,37 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
       w += so[c] * de[c];
       s[c] = so[c] + ds[c];
     }
-    R dg;
-    if constexpr (PHYS == PHYS_EP_MLP) {
-      dg = mlp_return<R>(p, de, so, ep, s, active);
-    } else {
-      R ep0 = ep;
-      dg = radial_return<R>(p, s, ep);
-      if (p.samples && dg > R(0)) record_sample<R>(p, de, so, ep0, s, ep);
-    }
+    const R ep0 = ep;
+    const R dg = radial_return<R>(p, s, ep);
+    if (p.samples && dg > R(0)) record_sample<R>(p, de, so, ep0, s, ep);
 #pragma unroll
     for (int c = 0; c < 6; c++) w += s[c] * de[c];
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-    if (active) store_state<R>(p, e, q, s, ep);
+    store_state<R>(p, e, q, s, ep);
   }
+}
+
+// trial stress s = sigma_n + D de; returns sigma_n : de (the first half of
+// the midpoint work)
+template <typename R>
+__device__ __forceinline__ R ep_trial(const Params &p, const R de[6], const R so[6], R s[6]) {
+  const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
+  R ds[6];
+  ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
+  ds[1] = c11 * de[1] + c12 * (de[0] + de[2]);
+  ds[2] = c11 * de[2] + c12 * (de[0] + de[1]);
+  ds[3] = c44 * de[3];
+  ds[4] = c44 * de[4];
+  ds[5] = c44 * de[5];
+  R w = R(0);
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    w += so[c] * de[c];
+    s[c] = so[c] + ds[c];
+  }
+  return w;
 }
 
 // f_i += wd * sigma . dN_i
@@ -163,15 +178,12 @@ __device__ __forceinline__ double corner_dt(const R e1[3], const R e2[3], const 
 constexpr double kTet4LengthFactor = 2.0;
 constexpr double kTet10LengthFactor = 4.35;
 
-// ---------------------------------------------------------------------------
-// tet4: B is constant, every quadrature point carries the same strain, so
-// the rule enters only through its weight sum (compat) and the element keeps
-// one stress point (ep).  Corner forces reach put() as soon as known.
-// ---------------------------------------------------------------------------
-template <typename R, int PHYS, bool PRE, class Get, class Put>
-__device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
-                                          const Put &put, EAcc &ea,
-                                          const PreState<R> *pre = nullptr, bool active = true) {
+// Kinematics of one tet4: inverse Jacobian, strain increment de (engineering
+// shear) and w detJ; counts a non-positive detJ and, for the elastoplastic
+// models, the element's stable dt into ea.
+template <typename R, int PHYS, class Get>
+__device__ __forceinline__ void tet4_kin(const Params &p, const Get &get, double dt, R Ji[9],
+                                         R de[6], R &wd, EAcc &ea) {
   R J[9];
   R u[3][3];
   {
@@ -188,7 +200,6 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
       }
     }
   }
-  R Ji[9];
   R detJ = inv3<R>(J, Ji);
   if (!(detJ > R(0))) ea.bad += 1.0;
   if constexpr (PHYS != PHYS_COMPAT) {
@@ -203,11 +214,18 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
 #pragma unroll
     for (int j = 0; j < 3; j++)
       L[i][j] = Ji[0 * 3 + j] * u[0][i] + Ji[1 * 3 + j] * u[1][i] + Ji[2 * 3 + j] * u[2][i];
-  R de[6] = {L[0][0], L[1][1], L[2][2], L[0][1] + L[1][0], L[1][2] + L[2][1],
-             L[0][2] + L[2][0]};
-  R wd = R(p.wsum) * detJ;
-  R s[6];
-  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre, active);
+  de[0] = L[0][0];
+  de[1] = L[1][1];
+  de[2] = L[2][2];
+  de[3] = L[0][1] + L[1][0];
+  de[4] = L[1][2] + L[2][1];
+  de[5] = L[0][2] + L[2][0];
+  wd = R(p.wsum) * detJ;
+}
+
+// corner forces w detJ sigma . dN_k of one tet4, handed to put() as known
+template <typename R, class Put>
+__device__ __forceinline__ void tet4_forces(const R Ji[9], const R s[6], R wd, const Put &put) {
   R f0[3] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int k = 1; k < 4; k++) {
@@ -219,6 +237,22 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
     put(k, fk);
   }
   put(0, f0);
+}
+
+// ---------------------------------------------------------------------------
+// tet4: B is constant, every quadrature point carries the same strain, so
+// the rule enters only through its weight sum (compat) and the element keeps
+// one stress point (ep).  Corner forces reach put() as soon as known.
+// ---------------------------------------------------------------------------
+template <typename R, int PHYS, bool PRE, class Get, class Put>
+__device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
+                                          const Put &put, EAcc &ea,
+                                          const PreState<R> *pre = nullptr) {
+  R Ji[9], de[6], wd;
+  tet4_kin<R, PHYS>(p, get, dt, Ji, de, wd, ea);
+  R s[6];
+  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre);
+  tet4_forces<R>(Ji, s, wd, put);
 }
 
 // ---------------------------------------------------------------------------

@@ -217,10 +217,7 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 // global load of the workgroup (stress state, local connectivity, the
 // contribution lists, node records, slots) is issued before the first
 // barrier.  LDS: node records [max_loc][8] doubles, corner forces
-// [max_el*4][3] R, and with the surrogate one MLP staging area per wave.
-// The surrogate's return map is wave-cooperative, so with it every lane runs
-// the element code: lanes past the block's last element repeat its first
-// element and store nothing.
+// [max_el*4][3] R.
 template <typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
@@ -236,12 +233,11 @@ k_elem_block4(Params p) {
   R *lf = (R *)(smem + 8 * p.max_loc);
   const double dt = p.ctl->dt;
 
-  constexpr bool UNIFORM = PHYS == PHYS_EP_MLP;
   const bool has_e = t < nel;
-  const int e = has_e ? es + t : es;
+  const int e = es + t;
   uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
   PreState<R> pre;
-  if (UNIFORM || has_e) {
+  if (has_e) {
     lc4 = *(const uint2 *)(p.conn_local + 4L * e);
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
     if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, 0, pre.s, pre.ep);
@@ -258,7 +254,7 @@ k_elem_block4(Params p) {
   __syncthreads();
 
   EAcc ea;
-  if (UNIFORM || has_e) {
+  if (has_e) {
     const int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
                       (int)(lc4.y >> 16)};
     const int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
@@ -272,13 +268,11 @@ k_elem_block4(Params p) {
       }
     };
     auto put = [&](int k, const R fk[3]) {
-      if (UNIFORM && !has_e) return;
       R *dst = lf + 3 * ps[k];
 #pragma unroll
       for (int c = 0; c < 3; c++) dst[c] = fk[c];
     };
-    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre, has_e);
-    if (UNIFORM && !has_e) ea = EAcc();
+    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
   }
   __syncthreads();
 
@@ -288,6 +282,186 @@ k_elem_block4(Params p) {
       f[0] += double(lf[3 * s + 0]);
       f[1] += double(lf[3 * s + 1]);
       f[2] += double(lf[3 * s + 2]);
+    }
+    if (t < nint) {
+      const double *d = lnode + 8 * t;
+      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
+      node_update<AUX>(p, g, f, x, v, d[3], dt, ea);
+    } else {
+      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+    }
+  }
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
+}
+
+// tet4, FP32, J2 with the MLP surrogate of the return map.  Same structure
+// as k_elem_block4, with the return map taken out of the per-thread element
+// code so the network's registers never add to the element's:
+//   phase 1  every element: kinematics and trial stress; elastic ones
+//            finish here exactly as in k_elem_block4;
+//   queue    yielding elements in thread order (per-wave ballots + offsets:
+//            deterministic);
+//   phase 2  only if the queue is not empty: the waves take the queue 64 at a
+//            time, rebuild each element's kinematics from the LDS node
+//            records, evaluate the network with the matrix cores (samples
+//            packed, see mlp_wave_infer), check the yield residual (exact
+//            return on failure), then the element's work, state and forces.
+// LDS: node records, corner forces, one MLP staging area per wave, the
+// queue and the per-wave counts.
+template <bool AUX, int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_elem_block4_mlp(Params p) {
+  using R = float;
+  extern __shared__ double smem[];
+  __shared__ uint16_t queue[NT];
+  __shared__ int wcnt[NT / 64];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int32_t *mt = p.meta + 8 * b;
+  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
+            loff = mt[7];
+  const int nloc = nint + nbnd;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  double *lnode = smem;
+  R *lf = (R *)(smem + 8 * p.max_loc);
+  const double dt = p.ctl->dt;
+
+  const bool has_e = t < nel;
+  const int e = es + t;
+  uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
+  PreState<R> pre;
+  if (has_e) {
+    lc4 = *(const uint2 *)(p.conn_local + 4L * e);
+    ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
+    load_state<R>(p, e, 0, pre.s, pre.ep);
+  }
+  const bool has_n = t < nloc;
+  int g = 0, c0 = 0, c1 = 0, slot = 0;
+  if (has_n) {
+    g = t < nint ? is + t : p.bnd_nodes[boff + t - nint];
+    c0 = p.csr_ptr[loff + t];
+    c1 = p.csr_ptr[loff + t + 1];
+    if (t >= nint) slot = p.bnd_slot[boff + t - nint];
+    stage_node(lnode + 8 * t, p.xm[g], p.vv[g]);
+  }
+  __syncthreads();
+
+  auto unpack4 = [](uint2 v, int o[4]) {
+    o[0] = (int)(v.x & 0xffffu);
+    o[1] = (int)(v.x >> 16);
+    o[2] = (int)(v.y & 0xffffu);
+    o[3] = (int)(v.y >> 16);
+  };
+  EAcc ea;
+  bool yielding = false;
+  if (has_e) {
+    int l[4], ps[4];
+    unpack4(lc4, l);
+    unpack4(ps4, ps);
+    auto get = [&](int k, double xk[3], double vk[3]) {
+      const double *d = lnode + 8 * l[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        xk[c] = d[c];
+        vk[c] = d[4 + c];
+      }
+    };
+    auto put = [&](int k, const R fk[3]) {
+      R *dst = lf + 3 * ps[k];
+#pragma unroll
+      for (int c = 0; c < 3; c++) dst[c] = fk[c];
+    };
+    R Ji[9], de[6], wd, s[6];
+    tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, ea);
+    R wk = ep_trial<R>(p, de, pre.s, s);
+    if (!(yield_overshoot<R>(p, s, pre.ep) > R(0))) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) wk += s[c] * de[c];
+      ea.wint += double(wd * R(0.5) * wk);
+      store_state<R>(p, e, 0, s, pre.ep);
+      tet4_forces<R>(Ji, s, wd, put);
+    } else {
+      yielding = true;
+    }
+  }
+  const uint64_t m = __ballot(yielding);
+  if (lane == 0) wcnt[w] = __popcll(m);
+  __syncthreads();
+  int total = 0, off = 0;
+#pragma unroll
+  for (int i = 0; i < NT / 64; i++) {
+    const int c = wcnt[i];
+    off += i < w ? c : 0;
+    total += c;
+  }
+  if (total > 0) {
+    if (yielding) queue[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)t;
+    __syncthreads();
+    for (int base = w * 64; base < total; base += NT) {
+      const int k = base + lane;
+      const bool act = k < total;
+      const int e2 = es + queue[act ? k : base];
+      const uint2 lc2 = *(const uint2 *)(p.conn_local + 4L * e2);
+      auto get = [&](int kk, double xk[3], double vk[3]) {
+        int l[4];
+        unpack4(lc2, l);
+        const double *d = lnode + 8 * l[kk];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          xk[c] = d[c];
+          vk[c] = d[4 + c];
+        }
+      };
+      // inputs, then the network: nothing but the outputs, ep and the
+      // element's indices stays live across the matrix-core code
+      float y[kMlpOut];
+      R ep;
+      {
+        R so[6], Ji[9], de[6], wd;
+        load_state<R>(p, e2, 0, so, ep);
+        EAcc again;  // counted in phase 1
+        tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, again);
+        float x[kMlpIn];
+        mlp_inputs<R>(*p.mlp, de, so, ep, x);
+        mlp_wave_infer(*p.mlp, x, y, mlp_stage(p), act, __ballot(act));
+      }
+      // the element again from LDS and the state (cached), then the return
+      R so[6], Ji[9], de[6], wd, s[6];
+      load_state<R>(p, e2, 0, so, ep);
+      EAcc again;
+      tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, again);
+      R wk = ep_trial<R>(p, de, so, s);
+      int how = MLP_ELASTIC;
+      const R q = yield_overshoot<R>(p, s, ep);
+      R dg = R(0);
+      if (act && q > R(0)) dg = mlp_apply<R>(p, y, q, ep, s, how);
+      mlp_count(p, how == MLP_ACCEPTED, how == MLP_FALLBACK);
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) wk += s[c] * de[c];
+        ea.wint += double(wd * R(0.5) * wk);
+        if (dg > R(0))
+          ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
+        store_state<R>(p, e2, 0, s, ep);
+        const uint2 ps2 = *(const uint2 *)(p.csr_pos + 4L * e2);
+        auto put = [&](int kk, const R fk[3]) {
+          int ps[4];
+          unpack4(ps2, ps);
+          R *dst = lf + 3 * ps[kk];
+#pragma unroll
+          for (int c = 0; c < 3; c++) dst[c] = fk[c];
+        };
+        tet4_forces<R>(Ji, s, wd, put);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (has_n) {
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int q = c0; q < c1; q++) {
+      f[0] += double(lf[3 * q + 0]);
+      f[1] += double(lf[3 * q + 1]);
+      f[2] += double(lf[3 * q + 2]);
     }
     if (t < nint) {
       const double *d = lnode + 8 * t;
@@ -768,11 +942,13 @@ __global__ void __launch_bounds__(256) k_stream_copy(const Node4 *__restrict__ s
 
 template <int NPE> constexpr int block_threads() { return NPE == 4 ? 384 : 128; }
 
-// minimum waves per SIMD the tet4 element kernel is compiled for: FP64 fits
-// 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU); the
-// surrogate's inference needs 128 (two per CU)
+// minimum waves per SIMD the tet4 element kernels are compiled for: FP64
+// fits 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU).
+// The surrogate kernel's LDS (MLP staging) allows four workgroups per CU,
+// i.e. six waves per SIMD, so it may use 80 VGPRs: with them its spill code
+// stays inside the network phase, off the elastic path.
 template <typename R, int PHYS> constexpr int tet4_waves() {
-  return PHYS == PHYS_EP_MLP ? 4 : std::is_same<R, double>::value ? 5 : 8;
+  return PHYS == PHYS_EP_MLP ? 6 : std::is_same<R, double>::value ? 5 : 8;
 }
 
 // LDS of the workgroup element kernels: node records, corner forces and,
@@ -804,7 +980,11 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
 template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
   constexpr int NT = block_threads<NPE>();
-  if constexpr (NPE == 4) {
+  if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
+    constexpr int W = tet4_waves<R, PHYS>();
+    if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block4_mlp<false, NT, W>), nblocks, NT, smem, s, p);
+  } else if constexpr (NPE == 4) {
     constexpr int W = tet4_waves<R, PHYS>();
     if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
     CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
@@ -873,7 +1053,12 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
       return hipErrorInvalidValue;
     } else {
       constexpr int NT = block_threads<NPE>();
-      if constexpr (NPE == 4) {
+      if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
+        constexpr int W = tet4_waves<R, PHYS>();
+        hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W>, smem);
+        if (e != hipSuccess) return e;
+        return lds_opt_in(k_elem_block4_mlp<false, NT, W>, smem);
+      } else if constexpr (NPE == 4) {
         constexpr int W = tet4_waves<R, PHYS>();
         hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
         if (e != hipSuccess) return e;

@@ -133,11 +133,6 @@ __device__ __forceinline__ void record_sample(const Params &p, const R de[6], co
   d[19] = float(ep1);
 }
 
-// Surrogate return map; defined in mlp_infer.h (wave-cooperative MFMA).
-template <typename R>
-__device__ R mlp_return(const Params &p, const R de[6], const R so[6], R &ep, R s[6],
-                        bool active);
-
 // outcome of one point of the surrogate return map
 enum : int { MLP_ELASTIC = 0, MLP_ACCEPTED = 1, MLP_FALLBACK = 2 };
 

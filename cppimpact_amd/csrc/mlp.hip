@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(64) k_mlp_forward(const MlpParams *P, const fl
   float x[kMlpIn], y[kMlpOut];
 #pragma unroll
   for (int k = 0; k < kMlpIn; k++) x[k] = s < n ? X[(long)s * 16 + k] : 0.f;
-  mlp_wave_infer(*P, x, y, stage);
+  mlp_wave_infer(*P, x, y, stage, s < n, __ballot(s < n));
   if (s < n)
 #pragma unroll
     for (int o = 0; o < kMlpOut; o++) Y[(long)s * 8 + o] = y[o];
@@ -327,8 +327,14 @@ __global__ void __launch_bounds__(64) k_mlp_return_points(Params p, int n, const
   s[3] = o[3] + c44 * d[3];
   s[4] = o[4] + c44 * d[4];
   s[5] = o[5] + c44 * d[5];
+  const float q = yield_overshoot<float>(p, s, e);
+  const bool yielding = act && q > 0.f;
+  float x[kMlpIn], y[kMlpOut];
+  mlp_inputs<float>(*p.mlp, d, o, e, x);
+  const uint64_t mask = __ballot(yielding);
+  if (mask) mlp_wave_infer(*p.mlp, x, y, mlp_stage(p), yielding, mask);
   int h = MLP_ELASTIC;
-  mlp_return_how<float>(p, d, o, e, s, act, h);
+  if (yielding) mlp_apply<float>(p, y, q, e, s, h);
   if (act) {
 #pragma unroll
     for (int c = 0; c < 6; c++) s_out[k * 6 + c] = s[c];
