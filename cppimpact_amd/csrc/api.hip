@@ -397,17 +397,16 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       return fail(c, CPI_ERR_INVALID, "block_elements above " + std::to_string(nt) + " for tet4");
     // LDS budget per workgroup: three (FP64) or four (FP32) tet4 workgroups,
     // two tet10 ones per 160 KiB CU, never above the device's limit; node
-    // records take 64 bytes each
-    const size_t rsz = c->prec == CPI_PREC_F32 ? 4 : 8;
+    // records take 56 bytes each (slot offsets budgeted for 64 contributions
+    // per node)
     const long share = npe == 4 ? (c->prec == CPI_PREC_F32 ? 40960 : 54272) : 81920;
     const long budget = std::min<long>((long)prop.sharedMemPerBlock, share);
-    const long fixed = (long)elem_block_smem(npe, c->prec, c->phys, 0, me);
-    long mnl = (budget - fixed - 64) / 64;
+    const long fixed = (long)elem_block_smem(npe, c->prec, c->phys, 0, me, 64);
+    long mnl = (budget - fixed - 16) / 56;
     if (mnl < 4L * npe)
       return fail(c, CPI_ERR_INVALID, "block_elements " + std::to_string(me) +
                                           " does not fit the LDS budget");
     int mn = (int)std::min<long>(mnl, npe == 4 ? nt : 1024);
-    (void)rsz;
     std::string e;
     if (!build_block_plan(c->ne, npe, c->conn_u.data(), c->nn, c->X0u.data(), me, mn, c->plan, e))
       return fail(c, CPI_ERR_INVALID, "partition: " + e);
@@ -494,32 +493,42 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   }
   if (c->scatter == CPI_SCATTER_BLOCK) {
     BlockPlan &B = c->plan;
-    int32_t *meta, *bn, *bs, *sp;
-    uint16_t *cl, *cp, *ci;
+    if (npe == 4 && B.max_local_nodes > elem_block_threads(4))
+      return fail(c, CPI_ERR_INVALID, "partition: a tet4 block has more nodes than threads");
+    int32_t *meta, *bn, *bs, *sp, *dn, *ds;
+    uint16_t *cl, *ci, *dg, *dof;
     Node4 *part;
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||
-        (rc = dalloc(c, &cp, B.csr_ptr.size())) || (rc = upload(c, cp, B.csr_ptr)) ||
         (rc = dalloc(c, &ci, B.csr_pos.size())) || (rc = upload(c, ci, B.csr_pos)) ||
         (rc = dalloc(c, &bn, B.bnd_nodes.size())) || (rc = upload(c, bn, B.bnd_nodes)) ||
         (rc = dalloc(c, &bs, B.bnd_slot.size())) || (rc = upload(c, bs, B.bnd_slot)) ||
         (rc = dalloc(c, &sp, B.slot_ptr.size())) || (rc = upload(c, sp, B.slot_ptr)) ||
+        (rc = dalloc(c, &dn, B.nid_pad.size())) || (rc = upload(c, dn, B.nid_pad)) ||
+        (rc = dalloc(c, &ds, B.slot_pad.size())) || (rc = upload(c, ds, B.slot_pad)) ||
+        (rc = dalloc(c, &dg, B.deg_pad.size())) || (rc = upload(c, dg, B.deg_pad)) ||
+        (rc = dalloc(c, &dof, B.off_pad.size())) || (rc = upload(c, dof, B.off_pad)) ||
         (rc = dalloc(c, &part, (size_t)std::max(B.n_slots, 1))))
       return rc;
     P.meta = meta;
     P.conn_local = cl;
-    P.csr_ptr = cp;
     P.csr_pos = ci;
     P.bnd_nodes = bn;
     P.bnd_slot = bs;
     P.slot_ptr = sp;
+    P.nid_pad = dn;
+    P.slot_pad = ds;
+    P.deg_pad = dg;
+    P.off_pad = dof;
+    P.ml = B.max_local_nodes;
+    P.kmax = B.kmax;
     P.partial = part;
     P.n_interior = B.n_interior;
     P.n_boundary = B.n_boundary;
     P.max_loc = B.max_local_nodes;
     P.max_el = B.max_block_elems;
-    c->smem = elem_block_smem(npe, c->prec, c->phys, P.max_loc, P.max_el);
-    P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.max_loc, P.max_el);
+    c->smem = elem_block_smem(npe, c->prec, c->phys, P.ml, P.max_el, P.kmax);
+    P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.ml, P.max_el, P.kmax);
     if (c->smem > (size_t)prop.sharedMemPerBlock)
       return fail(c, CPI_ERR_INVALID, "element block needs " + std::to_string(c->smem) +
                                           " bytes of LDS; lower block_elements");

@@ -69,12 +69,21 @@ struct Params {
   // workgroup plan (LDS-assembled scatter)
   const int32_t *meta;
   const uint16_t *conn_local;
-  const uint16_t *csr_ptr;   // per block: local node -> first force position
   const uint16_t *csr_pos;   // [E][npe]: corner -> force position in its block
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;
   const int32_t *slot_ptr;
   Node4 *partial;        // one force partial per slot (x, y, z, -)
+  // per-block node tables at a fixed stride ml (local node j of block b at
+  // b*ml + j), read without the block's meta: device node id (-1 past the
+  // block's nodes), partial slot (-1 for interior nodes), contribution
+  // count; and kmax+1 slot offsets off[k] per block (jagged-diagonal corner
+  // force slots: local node j's k-th contribution at off[k] + j)
+  const int32_t *nid_pad;
+  const int32_t *slot_pad;
+  const uint16_t *deg_pad;
+  const uint16_t *off_pad;
+  int ml, kmax;
   int n_interior, n_boundary, max_loc, max_el;
   // energy partial rows and control block
   double *epart;         // one row of EP_N doubles per wave of the step kernels

@@ -228,27 +228,63 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
   P.n_slots = P.slot_ptr[P.n_boundary];
   std::vector<int32_t> rank(P.n_boundary, 0);
 
-  // 6. per-block local numbering, local connectivity, contribution CSR
+  // 6. per-block local numbering.  Local nodes are sorted by their number of
+  // corner references in the block (descending; ties keep interior before
+  // boundary and device order).  Corner forces are stored jagged-diagonal:
+  // the k-th contribution (element order) of local node l sits at
+  // off[k] + l, off[k] = sum over k' < k of the count of nodes with more
+  // than k' contributions; a node's sum then reads consecutive addresses
+  // across the nodes of the block.
   P.nblocks = nb;
   P.meta.assign((size_t)nb * 8, 0);
   P.elem_perm.assign(ne, 0);
   P.conn_local.assign((size_t)ne * npe, 0);
   P.csr_pos.assign((size_t)ne * npe, 0);
-  P.csr_ptr.clear();
   P.bnd_nodes.clear();
   P.bnd_slot.clear();
   P.max_local_nodes = 0;
   P.max_block_elems = 0;
+  P.kmax = 0;
   std::vector<int32_t> local(nn, -1);
+  std::vector<std::vector<int32_t>> b_dev(nb), b_slot(nb), b_deg(nb), b_off(nb);
   for (int b = 0; b < nb; b++) {
     std::vector<int32_t> dev;  // device ids of the block's boundary nodes
     for (int32_t g : bnodes[b])
       if (nblk[g] > 1) dev.push_back(P.node_inv[g]);
     std::sort(dev.begin(), dev.end());
-    for (int j = 0; j < n_int[b]; j++) local[P.node_perm[int_start[b] + j]] = j;
-    for (size_t j = 0; j < dev.size(); j++) local[P.node_perm[dev[j]]] = n_int[b] + (int)j;
-    const int nl = n_int[b] + (int)dev.size();
+    // candidates in the old order: interior (device order), then boundary
+    std::vector<int32_t> cand;
+    for (int j = 0; j < n_int[b]; j++) cand.push_back(int_start[b] + j);
+    for (int32_t d : dev) cand.push_back(d);
+    const int nl = (int)cand.size();
     const int es = blk_start[b], nel = blk_start[b + 1] - blk_start[b];
+    for (int j = 0; j < nl; j++) local[P.node_perm[cand[j]]] = j;
+    std::vector<int32_t> deg(nl, 0);
+    for (int t = 0; t < nel; t++) {
+      const int32_t *c = conn + (size_t)npe * order[es + t];
+      for (int j = 0; j < npe; j++) deg[local[c[j]]]++;
+    }
+    std::vector<int32_t> ord(nl);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int c) { return deg[a] > deg[c]; });
+    std::vector<int32_t> dev_l(nl), slot_l(nl, -1), deg_l(nl);
+    for (int j = 0; j < nl; j++) {
+      const int o = ord[j];
+      dev_l[j] = cand[o];
+      deg_l[j] = deg[o];
+      local[P.node_perm[cand[o]]] = j;
+      if (cand[o] >= P.n_interior) {
+        const int bi = cand[o] - P.n_interior;
+        slot_l[j] = P.slot_ptr[bi] + rank[bi]++;
+      }
+    }
+    const int km = nl ? deg_l[0] : 0;
+    std::vector<int32_t> off(km + 1, 0);
+    for (int k = 0; k < km; k++) {
+      int nk = 0;
+      while (nk < nl && deg_l[nk] > k) nk++;
+      off[k + 1] = off[k] + nk;
+    }
     int32_t *m = &P.meta[(size_t)b * 8];
     m[0] = es;
     m[1] = nel;
@@ -257,32 +293,45 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
     m[4] = (int32_t)P.bnd_nodes.size();
     m[5] = (int32_t)dev.size();
     m[6] = es * npe;
-    m[7] = (int32_t)P.csr_ptr.size();
-    for (int32_t d : dev) {
-      int bi = d - P.n_interior;
-      P.bnd_nodes.push_back(d);
-      P.bnd_slot.push_back(P.slot_ptr[bi] + rank[bi]++);
-    }
-    std::vector<int32_t> cnt(nl + 1, 0);
+    m[7] = km;
+    for (int j = 0; j < nl; j++)
+      if (slot_l[j] >= 0) {
+        P.bnd_nodes.push_back(dev_l[j]);
+        P.bnd_slot.push_back(slot_l[j]);
+      }
+    std::vector<int32_t> seen(nl, 0);
     for (int t = 0; t < nel; t++) {
       int e = order[es + t];
       P.elem_perm[es + t] = e;
       for (int j = 0; j < npe; j++) {
-        int l = local[conn[(size_t)npe * e + j]];
+        const int l = local[conn[(size_t)npe * e + j]];
         P.conn_local[(size_t)(es + t) * npe + j] = (uint16_t)l;
-        cnt[l + 1]++;
+        P.csr_pos[(size_t)(es + t) * npe + j] = (uint16_t)(off[seen[l]++] + l);
       }
     }
-    for (int l = 0; l < nl; l++) cnt[l + 1] += cnt[l];
-    for (int l = 0; l <= nl; l++) P.csr_ptr.push_back((uint16_t)cnt[l]);
-    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
-    for (int t = 0; t < nel; t++)
-      for (int j = 0; j < npe; j++) {
-        int l = P.conn_local[(size_t)(es + t) * npe + j];
-        P.csr_pos[(size_t)(es + t) * npe + j] = (uint16_t)(pos[l]++);
-      }
     P.max_local_nodes = std::max(P.max_local_nodes, nl);
     P.max_block_elems = std::max(P.max_block_elems, nel);
+    P.kmax = std::max(P.kmax, km);
+    b_dev[b] = std::move(dev_l);
+    b_slot[b] = std::move(slot_l);
+    b_deg[b] = std::move(deg_l);
+    b_off[b] = std::move(off);
+  }
+  // fixed-stride tables: local node l of block b at b*ml + l
+  const int ml = P.max_local_nodes, ko = P.kmax + 1;
+  P.nid_pad.assign((size_t)nb * ml, -1);
+  P.slot_pad.assign((size_t)nb * ml, -1);
+  P.deg_pad.assign((size_t)nb * ml, 0);
+  P.off_pad.assign((size_t)nb * ko, 0);
+  for (int b = 0; b < nb; b++) {
+    for (size_t l = 0; l < b_dev[b].size(); l++) {
+      P.nid_pad[(size_t)b * ml + l] = b_dev[b][l];
+      P.slot_pad[(size_t)b * ml + l] = b_slot[b][l];
+      P.deg_pad[(size_t)b * ml + l] = (uint16_t)b_deg[b][l];
+    }
+    for (int k = 0; k < ko; k++)
+      P.off_pad[(size_t)b * ko + k] =
+          (uint16_t)b_off[b][std::min<size_t>(k, b_off[b].size() - 1)];
   }
   return true;
 }

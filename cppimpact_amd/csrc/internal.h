@@ -26,8 +26,9 @@ int greedy_coloring(int ne, int npe, const int32_t *conn, int nn,
 // Device element order: block order, Hilbert order inside a block.
 struct BlockPlan {
   int nblocks = 0;
-  int max_local_nodes = 0;  // over all blocks
+  int max_local_nodes = 0;  // over all blocks (ml)
   int max_block_elems = 0;
+  int kmax = 0;             // most contributions of one local node
   int n_interior = 0;       // device ids [0, n_interior) are interior nodes
   int n_boundary = 0;       // device ids [n_interior, n_interior+n_boundary)
   int n_slots = 0;
@@ -35,16 +36,21 @@ struct BlockPlan {
   std::vector<int32_t> node_perm;  // device node -> user node
   std::vector<int32_t> node_inv;   // user node -> device node
   // per block, 8 int32: elem_start, n_elem, int_start, n_int, bnd_off,
-  // n_bnd, csr_off, loc_off (offset of the block's n_loc+1 csr offsets)
+  // n_bnd, corner_off, kmax of the block
   std::vector<int32_t> meta;
   std::vector<uint16_t> conn_local;  // [E][npe] local node index
   std::vector<int32_t> bnd_nodes;    // per block boundary nodes (device id)
   std::vector<int32_t> bnd_slot;     // parallel to bnd_nodes: partial slot
-  // Corner forces of a block are stored grouped by local node: node l owns
-  // positions [csr_ptr[l], csr_ptr[l+1]) in element order, and corner j of
-  // element t writes to position csr_pos[t*npe + j].
-  std::vector<uint16_t> csr_ptr;     // per block n_loc+1 offsets
+  // Corner forces of a block are stored jagged-diagonal (local nodes sorted
+  // by contribution count): the k-th contribution of local node l, in
+  // element order, is at off[k] + l; corner j of element t writes to
+  // csr_pos[t*npe + j].
   std::vector<uint16_t> csr_pos;     // [E][npe] position of each corner force
+  // fixed-stride per-block tables, local node l of block b at b*ml + l:
+  std::vector<int32_t> nid_pad;      // device node id, -1 past the block's nodes
+  std::vector<int32_t> slot_pad;     // partial slot, -1 for interior nodes
+  std::vector<uint16_t> deg_pad;     // contribution count
+  std::vector<uint16_t> off_pad;     // per block kmax+1 offsets off[k]
   std::vector<int32_t> slot_ptr;     // per boundary node, n_boundary+1
 };
 

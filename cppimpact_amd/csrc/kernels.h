@@ -7,8 +7,8 @@
 
 namespace cpi {
 
-size_t elem_block_mlp_offset(int npe, int prec, int max_loc, int max_el);
-size_t elem_block_smem(int npe, int prec, int phys, int max_loc, int max_el);
+size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax);
+size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax);
 int elem_block_threads(int npe);
 int elem_direct_threads(int npe);
 

@@ -49,9 +49,25 @@ __device__ __forceinline__ double wave_min(double v) {
 // every step; the instantaneous ones (kinetic and penalty energy, contact
 // force, stable dt) are only reduced on the steps that end a batch, the only
 // ones whose rows are summed.  The row of a wave is base + its wave index in
-// the grid.
+// the grid.  The cumulative terms of the row are read when the kernel starts
+// (row_prefetch), so the flush at its end only writes.
+struct RowPre {
+  double wint, wpl, bad, wgrav;
+};
+
+__device__ __forceinline__ RowPre row_prefetch(const Params &p, int base) {
+  // wave-uniform address: the values land in scalar registers
+  const int row = __builtin_amdgcn_readfirstlane(base + (int)(threadIdx.x >> 6));
+  const double *r = p.epart + (long)row * EP_N;
+  return RowPre{r[EP_WINT], r[EP_WPL], r[EP_BAD], r[EP_WGRAV]};
+}
+
+// One wave's energy terms into its row: the cumulative ones added to what
+// the row held (pre), and with ALL the instantaneous ones (kinetic,
+// penalty, contact force, stable dt) stored.
 template <bool ALL>
-__device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc &ea) {
+__device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc &ea,
+                                           const RowPre &pre) {
   const double wint = wave_sum(ea.wint), wpl = wave_sum(ea.wpl), bad = wave_sum(ea.bad),
                wg = wave_sum(ea.wgrav);
   double ke = 0.0, pen = 0.0, fc = 0.0, md = 0.0;
@@ -63,10 +79,10 @@ __device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc
   }
   if ((threadIdx.x & 63) == 0) {
     double *r = p.epart + (long)(base + (threadIdx.x >> 6)) * EP_N;
-    r[EP_WINT] += wint;
-    r[EP_WPL] += wpl;
-    r[EP_BAD] += bad;
-    r[EP_WGRAV] += wg;
+    r[EP_WINT] = pre.wint + wint;
+    r[EP_WPL] = pre.wpl + wpl;
+    r[EP_BAD] = pre.bad + bad;
+    r[EP_WGRAV] = pre.wgrav + wg;
     if (ALL) {
       r[EP_MINDT] = md;
       r[EP_KE] = ke;
@@ -126,38 +142,65 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
   ea.wgrav += m * wg;
 }
 
-// LDS node record: (x, y, z, m, vx, vy, vz, -)
-__device__ __forceinline__ void stage_node(double *d, const Node4 &xm, const Node4 &vv) {
-  d[0] = xm.x;
-  d[1] = xm.y;
-  d[2] = xm.z;
-  d[3] = xm.w;
-  d[4] = vv.x;
-  d[5] = vv.y;
-  d[6] = vv.z;
+// A workgroup's LDS node records, structure of arrays of ml doubles each:
+// x, y, z, m, vx, vy, vz.  Lanes gathering random nodes read 8-byte words at
+// an 8-byte stride, so the reads spread over the LDS banks (records of 64
+// bytes put every node on one of four bank groups).
+struct LNodes {
+  double *d;
+  int ml;
+  __device__ __forceinline__ void stage(int j, const Node4 &xm, const Node4 &vv) const {
+    d[j] = xm.x;
+    d[ml + j] = xm.y;
+    d[2 * ml + j] = xm.z;
+    d[3 * ml + j] = xm.w;
+    d[4 * ml + j] = vv.x;
+    d[5 * ml + j] = vv.y;
+    d[6 * ml + j] = vv.z;
+  }
+  __device__ __forceinline__ void get(int l, double x[3], double v[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      x[c] = d[c * ml + l];
+      v[c] = d[(4 + c) * ml + l];
+    }
+  }
+  __device__ __forceinline__ double mass(int l) const { return d[3 * ml + l]; }
+};
+
+// bytes of the LDS regions of the workgroup element kernels: node records,
+// corner forces (jagged-diagonal slots), the slot offsets off[k]
+__host__ __device__ constexpr int lds_round16(long n) { return (int)(((n + 15) / 16) * 16); }
+__host__ __device__ constexpr int lds_node_bytes(int ml) { return lds_round16(56L * ml); }
+__host__ __device__ constexpr int lds_force_bytes(int rsz, int npe, int max_el) {
+  return lds_round16((long)rsz * 3 * npe * max_el);
 }
+__host__ __device__ constexpr int lds_off_bytes(int kmax) { return lds_round16(2L * (kmax + 1)); }
 
 // ---------------------------------------------------------------------------
 // LDS-assembled element pass, tet10: loops over elements and nodes.  LDS:
-// node records [max_loc][8] doubles, corner forces [max_el*NPE][3] R.
+// node records (LNodes), corner forces [max_el*NPE][3] R in jagged-diagonal
+// slots (local node j's k-th contribution at off[k] + j), off[k].
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, typename R, int PHYS, bool AUX, int NT>
 __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
   const int32_t *mt = p.meta + 8 * b;
-  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
-            coff = mt[6], loff = mt[7];
-  const int nloc = nint + nbnd;
-  (void)coff;
-  double *lnode = smem;
-  R *lf = (R *)(smem + 8 * p.max_loc);
+  const int es = mt[0], nel = mt[1];
+  const int ml = p.ml;
+  const int32_t *nid = p.nid_pad + (long)b * ml;
+  const LNodes ln{smem, ml};
+  R *lf = (R *)((char *)smem + lds_node_bytes(ml));
+  uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), NPE, p.max_el));
   const double dt = p.ctl->dt;
 
-  for (int j = threadIdx.x; j < nloc; j += NT) {
-    const int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
-    stage_node(lnode + 8 * j, p.xm[g], p.vv[g]);
+  for (int j = threadIdx.x; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g >= 0) ln.stage(j, p.xm[g], p.vv[g]);
   }
+  for (int k = threadIdx.x; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
   __syncthreads();
 
   EAcc ea;
@@ -165,14 +208,7 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
     const int e = es + t;
     const uint16_t *lc = p.conn_local + (long)e * NPE;
     const uint16_t *pos = p.csr_pos + (long)e * NPE;
-    auto get = [&](int k, double xk[3], double vk[3]) {
-      const double *d = lnode + 8 * lc[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        xk[c] = d[c];
-        vk[c] = d[4 + c];
-      }
-    };
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(lc[k], xk, vk); };
     // the element's corner slots belong to this thread alone: its forces
     // accumulate there point by point instead of in registers
 #pragma unroll
@@ -192,45 +228,56 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   }
   __syncthreads();
 
-  const uint16_t *cp = p.csr_ptr + loff;
-  for (int j = threadIdx.x; j < nloc; j += NT) {
+  for (int j = threadIdx.x; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g < 0) continue;
     double f[3] = {0.0, 0.0, 0.0};
-    const int s0 = cp[j], s1 = cp[j + 1];
-    for (int s = s0; s < s1; s++) {
-      f[0] += double(lf[3 * s + 0]);
-      f[1] += double(lf[3 * s + 1]);
-      f[2] += double(lf[3 * s + 2]);
+    const int deg = p.deg_pad[(long)b * ml + j];
+    for (int k = 0; k < deg; k++) {
+      const int q = joff[k] + j;
+      f[0] += double(lf[3 * q + 0]);
+      f[1] += double(lf[3 * q + 1]);
+      f[2] += double(lf[3 * q + 2]);
     }
-    if (j < nint) {
-      const double *d = lnode + 8 * j;
-      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
-      node_update<AUX>(p, is + j, f, x, v, d[3], dt, ea);
+    const int slot = p.slot_pad[(long)b * ml + j];
+    if (slot < 0) {
+      double x[3], v[3];
+      ln.get(j, x, v);
+      node_update<AUX>(p, g, f, x, v, ln.mass(j), dt, ea);
     } else {
-      p.partial[p.bnd_slot[boff + j - nint]] = Node4{f[0], f[1], f[2], 0.0};
+      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
     }
   }
-  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
 // tet4: the plan gives every workgroup at most NT elements and NT local
 // nodes, so each thread owns one element and at most one node, and every
 // global load of the workgroup (stress state, local connectivity, the
 // contribution lists, node records, slots) is issued before the first
-// barrier.  LDS: node records [max_loc][8] doubles, corner forces
-// [max_el*4][3] R.
+// barrier.  LDS: node records (LNodes), corner forces [max_el*4][3] R in
+// jagged-diagonal slots, the slot offsets off[k].
 template <typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int32_t *mt = p.meta + 8 * b;
-  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
-            coff = mt[6], loff = mt[7];
-  const int nloc = nint + nbnd;
   const int t = threadIdx.x;
-  (void)coff;
-  double *lnode = smem;
-  R *lf = (R *)(smem + 8 * p.max_loc);
+  // the node side needs only b: ids, slots and force ranges at a fixed
+  // stride, then the node records
+  int g = -1, slot = -1, deg = 0;
+  if (t < p.ml) {
+    const long o = (long)b * p.ml + t;
+    g = p.nid_pad[o];
+    slot = p.slot_pad[o];
+    deg = p.deg_pad[o];
+  }
+  const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
+  const int32_t *mt = p.meta + 8 * b;
+  const int es = mt[0], nel = mt[1];
+  const LNodes ln{smem, p.ml};
+  R *lf = (R *)((char *)smem + lds_node_bytes(p.ml));
+  uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 4, p.max_el));
   const double dt = p.ctl->dt;
 
   const bool has_e = t < nel;
@@ -242,15 +289,9 @@ k_elem_block4(Params p) {
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
     if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, 0, pre.s, pre.ep);
   }
-  const bool has_n = t < nloc;
-  int g = 0, c0 = 0, c1 = 0, slot = 0;
-  if (has_n) {
-    g = t < nint ? is + t : p.bnd_nodes[boff + t - nint];
-    c0 = p.csr_ptr[loff + t];
-    c1 = p.csr_ptr[loff + t + 1];
-    if (t >= nint) slot = p.bnd_slot[boff + t - nint];
-    stage_node(lnode + 8 * t, p.xm[g], p.vv[g]);
-  }
+  const bool has_n = g >= 0;
+  if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
+  if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
   __syncthreads();
 
   EAcc ea;
@@ -259,14 +300,7 @@ k_elem_block4(Params p) {
                       (int)(lc4.y >> 16)};
     const int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
                        (int)(ps4.y >> 16)};
-    auto get = [&](int k, double xk[3], double vk[3]) {
-      const double *d = lnode + 8 * l[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        xk[c] = d[c];
-        vk[c] = d[4 + c];
-      }
-    };
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
     auto put = [&](int k, const R fk[3]) {
       R *dst = lf + 3 * ps[k];
 #pragma unroll
@@ -278,20 +312,21 @@ k_elem_block4(Params p) {
 
   if (has_n) {
     double f[3] = {0.0, 0.0, 0.0};
-    for (int s = c0; s < c1; s++) {
-      f[0] += double(lf[3 * s + 0]);
-      f[1] += double(lf[3 * s + 1]);
-      f[2] += double(lf[3 * s + 2]);
+    for (int k = 0; k < deg; k++) {
+      const int q = joff[k] + t;
+      f[0] += double(lf[3 * q + 0]);
+      f[1] += double(lf[3 * q + 1]);
+      f[2] += double(lf[3 * q + 2]);
     }
-    if (t < nint) {
-      const double *d = lnode + 8 * t;
-      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
-      node_update<AUX>(p, g, f, x, v, d[3], dt, ea);
+    if (slot < 0) {
+      double x[3], v[3];
+      ln.get(t, x, v);
+      node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
     } else {
       p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
     }
   }
-  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
 // tet4, FP32, J2 with the MLP surrogate of the return map.  Same structure
@@ -316,13 +351,20 @@ k_elem_block4_mlp(Params p) {
   __shared__ uint16_t queue[NT];
   __shared__ int wcnt[NT / 64];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int32_t *mt = p.meta + 8 * b;
-  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
-            loff = mt[7];
-  const int nloc = nint + nbnd;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  double *lnode = smem;
-  R *lf = (R *)(smem + 8 * p.max_loc);
+  int g = -1, slot = -1, deg = 0;
+  if (t < p.ml) {
+    const long o = (long)b * p.ml + t;
+    g = p.nid_pad[o];
+    slot = p.slot_pad[o];
+    deg = p.deg_pad[o];
+  }
+  const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
+  const int32_t *mt = p.meta + 8 * b;
+  const int es = mt[0], nel = mt[1];
+  const LNodes ln{smem, p.ml};
+  R *lf = (R *)((char *)smem + lds_node_bytes(p.ml));
+  uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 4, p.max_el));
   const double dt = p.ctl->dt;
 
   const bool has_e = t < nel;
@@ -334,15 +376,9 @@ k_elem_block4_mlp(Params p) {
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
     load_state<R>(p, e, 0, pre.s, pre.ep);
   }
-  const bool has_n = t < nloc;
-  int g = 0, c0 = 0, c1 = 0, slot = 0;
-  if (has_n) {
-    g = t < nint ? is + t : p.bnd_nodes[boff + t - nint];
-    c0 = p.csr_ptr[loff + t];
-    c1 = p.csr_ptr[loff + t + 1];
-    if (t >= nint) slot = p.bnd_slot[boff + t - nint];
-    stage_node(lnode + 8 * t, p.xm[g], p.vv[g]);
-  }
+  const bool has_n = g >= 0;
+  if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
+  if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
   __syncthreads();
 
   auto unpack4 = [](uint2 v, int o[4]) {
@@ -357,14 +393,7 @@ k_elem_block4_mlp(Params p) {
     int l[4], ps[4];
     unpack4(lc4, l);
     unpack4(ps4, ps);
-    auto get = [&](int k, double xk[3], double vk[3]) {
-      const double *d = lnode + 8 * l[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        xk[c] = d[c];
-        vk[c] = d[4 + c];
-      }
-    };
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
     auto put = [&](int k, const R fk[3]) {
       R *dst = lf + 3 * ps[k];
 #pragma unroll
@@ -404,12 +433,7 @@ k_elem_block4_mlp(Params p) {
       auto get = [&](int kk, double xk[3], double vk[3]) {
         int l[4];
         unpack4(lc2, l);
-        const double *d = lnode + 8 * l[kk];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          xk[c] = d[c];
-          vk[c] = d[4 + c];
-        }
+        ln.get(l[kk], xk, vk);
       };
       // inputs, then the network: nothing but the outputs, ep and the
       // element's indices stays live across the matrix-core code
@@ -458,25 +482,27 @@ k_elem_block4_mlp(Params p) {
 
   if (has_n) {
     double f[3] = {0.0, 0.0, 0.0};
-    for (int q = c0; q < c1; q++) {
+    for (int k = 0; k < deg; k++) {
+      const int q = joff[k] + t;
       f[0] += double(lf[3 * q + 0]);
       f[1] += double(lf[3 * q + 1]);
       f[2] += double(lf[3 * q + 2]);
     }
-    if (t < nint) {
-      const double *d = lnode + 8 * t;
-      const double x[3] = {d[0], d[1], d[2]}, v[3] = {d[4], d[5], d[6]};
-      node_update<AUX>(p, g, f, x, v, d[3], dt, ea);
+    if (slot < 0) {
+      double x[3], v[3];
+      ln.get(t, x, v);
+      node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
     } else {
       p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
     }
   }
-  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea);
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
 template <bool AUX>
 __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
   const int i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * 4);
   const double dt = p.ctl->dt;
   EAcc ea;
   if (i < p.n_boundary) {
@@ -492,7 +518,7 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
     const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
-  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea);
+  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea, row);
   if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
 }
 
@@ -502,6 +528,7 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
 template <int NPE, int RULE, typename R, int PHYS, bool ATOMIC, int NT>
 __global__ void __launch_bounds__(NT) k_elem_direct(Params p) {
   const int t = blockIdx.x * NT + threadIdx.x;
+  const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * (NT / 64));
   const double dt = p.ctl->dt;
   EAcc ea;
   if (t < p.elem_count) {
@@ -529,12 +556,13 @@ __global__ void __launch_bounds__(NT) k_elem_direct(Params p) {
     };
     elem_core<NPE, RULE, R, PHYS>(p, get, dt, e, put, ea);
   }
-  wave_flush<true>(p, p.epart_base + blockIdx.x * (NT / 64), ea);
+  wave_flush<true>(p, p.epart_base + blockIdx.x * (NT / 64), ea, row);
 }
 
 template <bool AUX>
 __global__ void __launch_bounds__(256) k_node_all(Params p) {
   const int g = blockIdx.x * 256 + threadIdx.x;
+  const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * 4);
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
@@ -548,7 +576,7 @@ __global__ void __launch_bounds__(256) k_node_all(Params p) {
     const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
-  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea);
+  wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea, row);
   if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
 }
 
@@ -556,6 +584,7 @@ __global__ void __launch_bounds__(256) k_node_all(Params p) {
 // (dynamics.h:454-463): a0 = wall(X0) + g; v += dt/2 a0
 __global__ void __launch_bounds__(256) k_node_init(Params p) {
   const int g = blockIdx.x * 256 + threadIdx.x;
+  const RowPre row = row_prefetch(p, blockIdx.x * 4);
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
@@ -587,7 +616,7 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
     p.xprev[g] = xm;
     ea.ke = 0.5 * m * ke;
   }
-  wave_flush<true>(p, blockIdx.x * 4, ea);
+  wave_flush<true>(p, blockIdx.x * 4, ea, row);
 }
 
 // Energy rows are summed in two fixed-order stages: kEredBlocks workgroups
@@ -664,18 +693,22 @@ __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rh
   extern __shared__ double smem[];
   const int b = blockIdx.x;
   const int32_t *mt = p.meta + 8 * b;
-  const int es = mt[0], nel = mt[1], is = mt[2], nint = mt[3], boff = mt[4], nbnd = mt[5],
-            coff = mt[6], loff = mt[7];
-  const int nloc = nint + nbnd;
-  double *lnode = smem;
-  double *lm = smem + 4 * p.max_loc;
-  for (int j = threadIdx.x; j < nloc; j += NT) {
-    const int g = j < nint ? is + j : p.bnd_nodes[boff + j - nint];
-    const Node4 q = p.xm[g];
-    lnode[4 * j + 0] = q.x;
-    lnode[4 * j + 1] = q.y;
-    lnode[4 * j + 2] = q.z;
+  const int es = mt[0], nel = mt[1];
+  const int ml = p.ml;
+  const int32_t *nid = p.nid_pad + (long)b * ml;
+  double *lx = smem;               // x, y, z: ml each
+  double *lm = smem + 3 * ml;      // corner masses, jagged-diagonal slots
+  uint16_t *joff = (uint16_t *)(lm + (long)p.max_el * NPE);
+  for (int j = threadIdx.x; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g >= 0) {
+      const Node4 q = p.xm[g];
+      lx[j] = q.x;
+      lx[ml + j] = q.y;
+      lx[2 * ml + j] = q.z;
+    }
   }
+  for (int k = threadIdx.x; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
   __syncthreads();
   for (int t = threadIdx.x; t < nel; t += NT) {
     const int e = es + t;
@@ -684,22 +717,24 @@ __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rh
     for (int k = 0; k < NPE; k++) {
       const int l = p.conn_local[(long)e * NPE + k];
 #pragma unroll
-      for (int c = 0; c < 3; c++) xs[k][c] = lnode[4 * l + c];
+      for (int c = 0; c < 3; c++) xs[k][c] = lx[c * ml + l];
     }
     elem_mass<NPE, RULE>(xs, kind, rho, me);
 #pragma unroll
     for (int k = 0; k < NPE; k++) lm[p.csr_pos[(long)e * NPE + k]] = me[k];
   }
   __syncthreads();
-  (void)coff;
-  const uint16_t *cp = p.csr_ptr + loff;
-  for (int j = threadIdx.x; j < nloc; j += NT) {
-    double s = 0.0;
-    for (int q = cp[j]; q < cp[j + 1]; q++) s += lm[q];
-    if (j < nint)
-      p.m[is + j] = s;
+  for (int j = threadIdx.x; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g < 0) continue;
+    double sum = 0.0;
+    const int deg = p.deg_pad[(long)b * ml + j];
+    for (int k = 0; k < deg; k++) sum += lm[joff[k] + j];
+    const int slot = p.slot_pad[(long)b * ml + j];
+    if (slot < 0)
+      p.m[g] = sum;
     else
-      p.partial[p.bnd_slot[boff + j - nint]].x = s;
+      p.partial[slot].x = sum;
   }
 }
 
@@ -953,14 +988,14 @@ template <typename R, int PHYS> constexpr int tet4_waves() {
 
 // LDS of the workgroup element kernels: node records, corner forces and,
 // with the surrogate, one MLP staging area per wave
-size_t elem_block_mlp_offset(int npe, int prec, int max_loc, int max_el) {
-  const size_t rsz = prec == 1 ? 4 : 8;
-  return 8 * 8 * (size_t)max_loc + ((rsz * 3 * npe * max_el + 15) / 16) * 16;
+size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax) {
+  const int rsz = prec == 1 ? 4 : 8;
+  return (size_t)lds_node_bytes(ml) + lds_force_bytes(rsz, npe, max_el) + lds_off_bytes(kmax);
 }
 
-size_t elem_block_smem(int npe, int prec, int phys, int max_loc, int max_el) {
+size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax) {
   const int nt = npe == 4 ? block_threads<4>() : block_threads<10>();
-  size_t s = elem_block_mlp_offset(npe, prec, max_loc, max_el);
+  size_t s = elem_block_mlp_offset(npe, prec, ml, max_el, kmax);
   if (phys == PHYS_EP_MLP) s += (size_t)kMlpStageBytes * (nt / 64);
   return s;
 }
@@ -1124,7 +1159,7 @@ static hipError_t mass_rule_t(const Params &p, int scatter, int kind, double rho
                               hipStream_t s) {
   if (scatter == 0) {
     constexpr int NT = NPE == 4 ? 512 : 128;
-    const size_t smem = 8 * (4 * (size_t)p.max_loc + (size_t)p.max_el * NPE);
+    const size_t smem = 8 * (3 * (size_t)p.ml + (size_t)p.max_el * NPE) + lds_off_bytes(p.kmax);
     if (smem > 65536) return hipErrorInvalidValue;  // sized by the plan to stay below
     hipLaunchKernelGGL((k_mass_block<NPE, RULE, NT>), dim3(nblocks), dim3(NT), smem, s, p, kind,
                        rho);
