@@ -40,7 +40,8 @@ class CpiError(RuntimeError):
 class MeshDesc(C.Structure):
     _fields_ = [("num_nodes", C.c_int32), ("num_elements", C.c_int32),
                 ("nodes_per_element", C.c_int32), ("xloc", _dp), ("element_nodes", _ip),
-                ("num_slave_nodes", C.c_int32), ("slave_nodes", _ip)]
+                ("num_slave_nodes", C.c_int32), ("slave_nodes", _ip),
+                ("num_fixed_nodes", C.c_int32), ("fixed_nodes", _ip)]
 
 
 class Material(C.Structure):
@@ -56,7 +57,7 @@ class Options(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "physics", "precision", "scatter", "quadrature", "mass", "stress_update",
         "use_graph", "track_energy", "contact_slave_only", "block_elements", "device",
-        "reserved0")] + [("gravity", C.c_double * 3), ("mlp_tolerance", C.c_double)]
+        "fixed_bc")] + [("gravity", C.c_double * 3), ("mlp_tolerance", C.c_double)]
 
 
 class Info(C.Structure):

@@ -296,6 +296,7 @@ extern "C" void cpi_default_options(cpi_options *o) {
   o->use_graph = 1;
   o->track_energy = 1;
   o->contact_slave_only = 0;
+  o->fixed_bc = 0;
   o->block_elements = 0;
   o->device = 0;
   o->gravity[0] = 0.0;
@@ -472,17 +473,25 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     if ((rc = dalloc(c, &d, cd.size())) || (rc = upload(c, d, cd))) return rc;
     P.conn = d;
   }
-  if (opt->contact_slave_only) {
+  // per-node flags in device node order from a list of user node ids
+  auto node_mask = [&](int n, const int32_t *ids, const char *what, const uint8_t **out) {
     std::vector<uint8_t> mask(nn, 0);
-    for (int i = 0; i < m->num_slave_nodes; i++) {
-      int s = m->slave_nodes ? m->slave_nodes[i] : -1;
-      if (s < 0 || s >= nn) return fail(c, CPI_ERR_MESH, "slave node out of range");
+    for (int i = 0; i < n; i++) {
+      const int s = ids ? ids[i] : -1;
+      if (s < 0 || s >= nn) return fail(c, CPI_ERR_MESH, std::string(what) + " node out of range");
       mask[c->ninv[s]] = 1;
     }
     uint8_t *d = nullptr;
-    if ((rc = dalloc(c, &d, mask.size())) || (rc = upload(c, d, mask))) return rc;
-    P.contact_mask = d;
-  }
+    int r = dalloc(c, &d, mask.size());
+    if (!r) r = upload(c, d, mask);
+    if (!r) *out = d;
+    return r;
+  };
+  if (opt->contact_slave_only &&
+      (rc = node_mask(m->num_slave_nodes, m->slave_nodes, "slave", &P.contact_mask)))
+    return rc;
+  if (opt->fixed_bc && (rc = node_mask(m->num_fixed_nodes, m->fixed_nodes, "fixed", &P.fixed_mask)))
+    return rc;
   if (c->phys != CPI_PHYSICS_COMPAT) {
     const size_t rs = c->prec == CPI_PREC_F32 ? 4 : 8;
     char *s = nullptr, *q = nullptr;

@@ -48,6 +48,7 @@ struct Params {
   int wall_on, wall_dim, wall_normal, pad0;
   double wall_loc, wall_k, wall_nk;
   const uint8_t *contact_mask;  // slave-only contact, or null for all nodes
+  const uint8_t *fixed_mask;    // held nodes (fixed set), or null for none
   double grav[3];
   // quadrature points (tet10) and the weight sum (tet4)
   double qw[kMaxQ];

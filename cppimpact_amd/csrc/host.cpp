@@ -739,12 +739,14 @@ extern "C" int cpi_inp_write(const char *path, const cpi_mesh_desc *m) {
       std::fprintf(f, ", %d", m->element_nodes[(long)npe * e + j] + 1);
     std::fprintf(f, "\n");
   }
-  if (m->num_slave_nodes > 0 && m->slave_nodes) {
-    std::fprintf(f, "*Nset, Nset=slave\n");
-    for (int i = 0; i < m->num_slave_nodes; i++)
-      std::fprintf(f, "%d%s", m->slave_nodes[i] + 1,
-                   (i % 16 == 15 || i + 1 == m->num_slave_nodes) ? "\n" : ", ");
-  }
+  auto nset = [&](const char *name, int n, const int32_t *ids) {
+    if (n <= 0 || !ids) return;
+    std::fprintf(f, "*Nset, Nset=%s\n", name);
+    for (int i = 0; i < n; i++)
+      std::fprintf(f, "%d%s", ids[i] + 1, (i % 16 == 15 || i + 1 == n) ? "\n" : ", ");
+  };
+  nset("fixed", m->num_fixed_nodes, m->fixed_nodes);
+  nset("slave", m->num_slave_nodes, m->slave_nodes);
   int rc = std::fclose(f);
   if (rc != 0) {
     set_global_error(std::string("cpi_inp_write: write failed: ") + path);

@@ -95,6 +95,8 @@ __device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc
 // ---------------------------------------------------------------------------
 // node update: a = -f/m + wall + g; x += du; v += dt a  (dynamics.h:497-503,
 // wall.h:37-44, dynamics_kernels.h:150-168).  The wall sees x + du = x_{n+1}.
+// A held node (fixed set, when enabled) has a = 0 and no contact; its
+// velocity is zero from the start, so it stays where it is.
 // ---------------------------------------------------------------------------
 template <bool AUX>
 __device__ __forceinline__ void node_update(const Params &p, int g, const double f[3],
@@ -107,7 +109,8 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
     du[c] = dt * v[c];
     a[c] = (m != 0.0) ? minv * (-f[c]) : 0.0;
   }
-  if (p.wall_on && (!p.contact_mask || p.contact_mask[g])) {
+  const bool held = p.fixed_mask && p.fixed_mask[g];
+  if (p.wall_on && !held && (!p.contact_mask || p.contact_mask[g])) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       if (c == p.wall_dim) {
@@ -124,7 +127,7 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
   double ke = 0.0, wg = 0.0, xn[3], vn[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    a[c] += p.grav[c];
+    a[c] = held ? 0.0 : a[c] + p.grav[c];
     vn[c] = v[c] + dt * a[c];
     double vi = vn[c] - 0.5 * dt * a[c];
     ke += vi * vi;
@@ -508,13 +511,30 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
   if (i < p.n_boundary) {
     const int g = p.n_interior + i;
     const Node4 xm = p.xm[g], vv = p.vv[g];
+    // partials in slot order, loaded four at a time
     double f[3] = {0.0, 0.0, 0.0};
-    for (int s = p.slot_ptr[i]; s < p.slot_ptr[i + 1]; s++) {
-      const Node4 q = p.partial[s];
+    const int s1 = p.slot_ptr[i + 1];
+    int s = p.slot_ptr[i];
+    auto add = [&](const Node4 &q) {
       f[0] += q.x;
       f[1] += q.y;
       f[2] += q.z;
+    };
+    for (; s + 4 <= s1; s += 4) {
+      const Node4 q0 = p.partial[s], q1 = p.partial[s + 1], q2 = p.partial[s + 2],
+                  q3 = p.partial[s + 3];
+      add(q0);
+      add(q1);
+      add(q2);
+      add(q3);
     }
+    if (s + 2 <= s1) {
+      const Node4 q0 = p.partial[s], q1 = p.partial[s + 1];
+      add(q0);
+      add(q1);
+      s += 2;
+    }
+    if (s < s1) add(p.partial[s]);
     const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
@@ -590,9 +610,11 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
   if (g < p.nn) {
     const Node4 xm = p.xm[g], vv = p.vv[g];
     const double m = xm.w;
-    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
+    const bool held = p.fixed_mask && p.fixed_mask[g];
+    const double x[3] = {xm.x, xm.y, xm.z};
+    const double v[3] = {held ? 0.0 : vv.x, held ? 0.0 : vv.y, held ? 0.0 : vv.z};
     double a[3] = {0.0, 0.0, 0.0}, ke = 0.0, vn[3];
-    if (p.wall_on && (!p.contact_mask || p.contact_mask[g])) {
+    if (p.wall_on && !held && (!p.contact_mask || p.contact_mask[g])) {
 #pragma unroll
       for (int c = 0; c < 3; c++)
         if (c == p.wall_dim) {
@@ -606,7 +628,7 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      a[c] += p.grav[c];
+      a[c] = held ? 0.0 : a[c] + p.grav[c];
       ke += v[c] * v[c];
       vn[c] = v[c] + 0.5 * dt * a[c];
     }

@@ -63,7 +63,7 @@ class Mesh:
         return 0
 
     @classmethod
-    def from_arrays(cls, xloc, element_nodes, slave_nodes=None):
+    def from_arrays(cls, xloc, element_nodes, slave_nodes=None, fixed_nodes=None):
         conn = np.ascontiguousarray(element_nodes, dtype=np.int32)
         m = cls(conn.shape[1])
         m.xloc = np.ascontiguousarray(xloc, dtype=np.float64).reshape(-1, 3)
@@ -72,6 +72,9 @@ class Mesh:
         if slave_nodes is not None:
             m.slave_nodes = np.ascontiguousarray(slave_nodes, dtype=np.int32)
             m.num_slave_nodes = m.slave_nodes.size
+        if fixed_nodes is not None:
+            m.fixed_nodes = np.ascontiguousarray(fixed_nodes, dtype=np.int32)
+            m.num_fixed_nodes = m.fixed_nodes.size
         return m
 
     @classmethod
@@ -92,12 +95,15 @@ class Mesh:
         L.check(L.lib().cpi_inp_write(filename.encode(), C.byref(d)))
 
     def _desc(self, slave=None):
+        fixed = self.fixed_nodes if len(self.fixed_nodes) else None
         self._keep = (np.ascontiguousarray(self.xloc, dtype=np.float64),
                       np.ascontiguousarray(self.element_nodes, dtype=np.int32),
-                      None if slave is None else np.ascontiguousarray(slave, dtype=np.int32))
-        x, conn, sl = self._keep
+                      None if slave is None else np.ascontiguousarray(slave, dtype=np.int32),
+                      None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32))
+        x, conn, sl, fx = self._keep
         return L.MeshDesc(x.shape[0], conn.shape[0], conn.shape[1], L.dptr(x), L.iptr(conn),
-                          0 if sl is None else sl.size, L.iptr(sl) if sl is not None else None)
+                          0 if sl is None else sl.size, L.iptr(sl) if sl is not None else None,
+                          0 if fx is None else fx.size, L.iptr(fx) if fx is not None else None)
 
 
 class BaseMaterial:

@@ -75,6 +75,8 @@ class Dynamics {
       d.num_slave_nodes = wall->num_slave_nodes;
       d.slave_nodes = wall->slave_node_indices;
     }
+    d.num_fixed_nodes = mesh->num_fixed_nodes;
+    d.fixed_nodes = mesh->fixed_nodes;
     cpi_material m{};
     m.E = material->E;
     m.rho = material->rho;

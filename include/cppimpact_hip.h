@@ -101,6 +101,9 @@ typedef struct cpi_mesh_desc {
   int32_t num_slave_nodes;       /* Mesh::slave_nodes; used only when
                                     cpi_options.contact_slave_only is set */
   const int32_t *slave_nodes;
+  int32_t num_fixed_nodes;       /* Mesh::fixed_nodes (mesh.h:270-298); used
+                                    only when cpi_options.fixed_bc is set */
+  const int32_t *fixed_nodes;
 } cpi_mesh_desc;
 
 /* BaseMaterial{E, rho, nu} + Elastoplastic{beta, H, Y0}
@@ -133,7 +136,10 @@ typedef struct cpi_options {
   int32_t contact_slave_only; /* 0 = every node, as the reference does */
   int32_t block_elements;   /* elements per workgroup for SCATTER_BLOCK, 0 = auto */
   int32_t device;           /* HIP device ordinal */
-  int32_t reserved0;
+  int32_t fixed_bc;         /* 1: hold the mesh's fixed nodes (zero initial
+                               velocity, zero acceleration, no contact);
+                               0 = none, as the reference (which collects
+                               the set and never applies it) */
   double gravity[3];        /* default (0, 0, -9.81): dynamics_kernels.h:165-167 */
   double mlp_tolerance;     /* default 1e-3 */
 } cpi_options;
@@ -405,7 +411,8 @@ int cpi_inp_read(const char *path, int32_t nodes_per_element,
                  int32_t *fixed_nodes, int32_t *slave_nodes, char *names);
 
 /* Writes an .inp the reader above (and the reference's) accepts: *Node,
-   *Element (C3D4/C3D10), and an Nset=slave holding the given nodes. */
+   *Element (C3D4/C3D10), and Nset=fixed / Nset=slave holding the given
+   nodes (each only when not empty). */
 int cpi_inp_write(const char *path, const cpi_mesh_desc *mesh);
 
 #ifdef __cplusplus

@@ -636,7 +636,8 @@ static void update(const orc_problem *p, orc_state *s, const double *du,
   const int dim = p->wall_dim;
   const double nk = p->wall_normal * p->wall_stiffness;
   for (int i = 0; i < nn; i++) {
-    if (p->wall_enabled) {
+    const int held = p->fixed_mask && p->fixed_mask[i];
+    if (p->wall_enabled && !held && (!p->contact_mask || p->contact_mask[i])) {
       double pos = s->x[3 * i + dim] + du[3 * i + dim];
       double dist = (pos - p->wall_location) * p->wall_normal;
       if (dist < 0.0) s->a[3 * i + dim] += -1 * (1 / s->m[i]) * dist * nk;
@@ -644,6 +645,7 @@ static void update(const orc_problem *p, orc_state *s, const double *du,
     s->a[3 * i + 0] += p->gravity[0];
     s->a[3 * i + 1] += p->gravity[1];
     s->a[3 * i + 2] += p->gravity[2];
+    if (held) s->a[3 * i + 0] = s->a[3 * i + 1] = s->a[3 * i + 2] = 0.0;
   }
 }
 
@@ -654,7 +656,8 @@ static void energy_row(const orc_problem *p, const orc_state *s,
   for (int i = 0; i < p->num_nodes; i++) {
     const double *v = vel + 3 * i;
     ke += 0.5 * s->m[i] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    if (p->wall_enabled) {
+    const int held = p->fixed_mask && p->fixed_mask[i];
+    if (p->wall_enabled && !held && (!p->contact_mask || p->contact_mask[i])) {
       double d = (s->x[3 * i + p->wall_dim] - p->wall_location) * p->wall_normal;
       if (d < 0.0) {
         pen += 0.5 * k * d * d;
@@ -690,6 +693,9 @@ int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
   memcpy(s->x, p->X0, sizeof(double) * nd);
   memcpy(s->Xref, p->X0, sizeof(double) * nd);
   memcpy(s->v, p->v0, sizeof(double) * nd);
+  if (p->fixed_mask)
+    for (int i = 0; i < nn; i++)
+      if (p->fixed_mask[i]) s->v[3 * i] = s->v[3 * i + 1] = s->v[3 * i + 2] = 0.0;
   memset(s->a, 0, sizeof(double) * nd);
   memset(s->m, 0, sizeof(double) * nn);
   memset(s->v_i, 0, sizeof(double) * nd);
@@ -702,7 +708,7 @@ int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
   double e0 = 0.0;
   if (s->energy) {
     double *r = s->energy;
-    energy_row(p, s, p->v0, r);
+    energy_row(p, s, s->v, r);
     r[ORC_E_INTERNAL] = wint;
     r[ORC_E_PLASTIC] = wpl;
     r[ORC_E_GRAVITY] = 0.0;

@@ -125,6 +125,12 @@ typedef struct orc_problem {
   double wall_location, wall_stiffness;
   double gravity[3];
   double dt;
+  /* per-node flags, NULL = none: contact_mask limits the wall to the flagged
+     nodes (the Wall's slave set); fixed_mask holds the flagged nodes (the
+     mesh's fixed set: zero initial velocity, zero acceleration, no contact).
+     The reference collects both sets (mesh.h:270-329) and applies neither. */
+  const unsigned char *contact_mask;
+  const unsigned char *fixed_mask;
 } orc_problem;
 
 typedef struct orc_state {

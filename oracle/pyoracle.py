@@ -32,6 +32,7 @@ class Problem(C.Structure):
         ("wall_enabled", C.c_int), ("wall_dim", C.c_int), ("wall_normal", C.c_int),
         ("wall_location", C.c_double), ("wall_stiffness", C.c_double),
         ("gravity", C.c_double * 3), ("dt", C.c_double),
+        ("contact_mask", C.POINTER(C.c_ubyte)), ("fixed_mask", C.POINTER(C.c_ubyte)),
     ]
 
 
@@ -208,7 +209,10 @@ class Solver:
 
     def __init__(self, X0, conn, v0, *, E, nu, rho, sigma_y=276e6, H=700e6,
                  physics=COMPAT, quad_rule=None, mass_kind=None, wall=None,
-                 gravity=(0.0, 0.0, -9.81), dt=1e-7):
+                 gravity=(0.0, 0.0, -9.81), dt=1e-7, contact_nodes=None, fixed_nodes=None):
+        """contact_nodes: node ids the wall acts on (None: all nodes);
+        fixed_nodes: node ids held in place (zero initial velocity and
+        acceleration, no contact)."""
         self.X0 = f64(X0).reshape(-1, 3)
         self.conn = np.ascontiguousarray(conn, dtype=np.int32)
         self.npe = self.conn.shape[1]
@@ -233,6 +237,15 @@ class Solver:
             p.wall_stiffness = float(wall["stiffness"])
         p.gravity = (C.c_double * 3)(*gravity)
         p.dt = dt
+        self.masks = []
+        for name, ids in (("contact_mask", contact_nodes), ("fixed_mask", fixed_nodes)):
+            if ids is None:
+                setattr(p, name, None)
+                continue
+            m = np.zeros(self.X0.shape[0], dtype=np.uint8)
+            m[np.asarray(ids, dtype=np.int64)] = 1
+            self.masks.append(m)
+            setattr(p, name, m.ctypes.data_as(C.POINTER(C.c_ubyte)))
         self.p = p
         self.nq = lib().orc_quad_npts(quad_rule)
 

@@ -19,8 +19,9 @@ def wall_k(side, n):
     return 10.0 * E * side / n
 
 
-def run_gpu(x, conn, v0, dt, nsteps, wall_stiffness=None, wall_location=0.0, **opts):
-    mesh = ci.Mesh.from_arrays(x, conn)
+def run_gpu(x, conn, v0, dt, nsteps, wall_stiffness=None, wall_location=0.0,
+            slave_nodes=None, fixed_nodes=None, **opts):
+    mesh = ci.Mesh.from_arrays(x, conn, slave_nodes=slave_nodes, fixed_nodes=fixed_nodes)
     mat = ci.Elastoplastic(E, RHO, NU, 1.0, H, SY)
     wall = None
     if wall_stiffness is not None:

@@ -173,7 +173,8 @@ def test_inp_missing_file_returns_one_like_the_reference(tmp_path):
 
 def test_inp_write_read_round_trip(tmp_path):
     x, conn = ci.box_mesh(3, 2, 2, 0.3, 0.2, 0.2, quadratic=True, jitter=0.1, seed=1)
-    m = ci.Mesh.from_arrays(x, conn, slave_nodes=np.arange(0, x.shape[0], 7, dtype=np.int32))
+    m = ci.Mesh.from_arrays(x, conn, slave_nodes=np.arange(0, x.shape[0], 7, dtype=np.int32),
+                            fixed_nodes=np.arange(1, x.shape[0], 5, dtype=np.int32))
     p = str(tmp_path / "m.inp")
     m.write_inp(p)
     r = ci.Mesh(10)
@@ -181,6 +182,8 @@ def test_inp_write_read_round_trip(tmp_path):
     assert np.array_equal(r.xloc, x)  # %.17g round-trips doubles
     assert np.array_equal(r.element_nodes, conn)
     assert np.array_equal(r.slave_nodes, m.slave_nodes)
+    assert np.array_equal(r.fixed_nodes, m.fixed_nodes)
+    assert r.node_set_names == ["fixed", "slave"]
 
 
 def test_loop_iteration_count_follows_float_accumulation():
