@@ -239,6 +239,8 @@ int reset_state(cpi_ctx *c) {
   if (c->phys != CPI_PHYSICS_COMPAT) {
     HIPCHK(c, hipMemsetAsync(c->P.sig, 0, rs * 6 * c->nq * (size_t)c->ne, c->stream));
     HIPCHK(c, hipMemsetAsync(c->P.epbar, 0, rs * c->nq * (size_t)c->ne, c->stream));
+    if (c->P.yielded)
+      HIPCHK(c, hipMemsetAsync(c->P.yielded, 0, 8 * (size_t)c->P.ywords * c->nblocks, c->stream));
   }
   HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -536,6 +538,14 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     P.n_boundary = B.n_boundary;
     P.max_loc = B.max_local_nodes;
     P.max_el = B.max_block_elems;
+    // the yielded bits: tet4, exact return (the surrogate kernel reads ep
+    // always and keeps no bits)
+    if (npe == 4 && c->phys == CPI_PHYSICS_EP) {
+      unsigned long long *yw = nullptr;
+      P.ywords = (B.max_block_elems + 63) / 64;
+      if ((rc = dalloc(c, &yw, (size_t)P.ywords * B.nblocks))) return rc;
+      P.yielded = yw;
+    }
     c->smem = elem_block_smem(npe, c->prec, c->phys, P.ml, P.max_el, P.kmax);
     P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.ml, P.max_el, P.kmax);
     if (c->smem > (size_t)prop.sharedMemPerBlock)

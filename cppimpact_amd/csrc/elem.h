@@ -55,22 +55,38 @@ __device__ __forceinline__ void load_state(const Params &p, int e, int q, R s[6]
   ep = P[(long)q * ne + e];
 }
 
+// ep is written only when it changed (a plastic step): elastic steps store
+// the stress alone
 template <typename R>
-__device__ __forceinline__ void store_state(const Params &p, int e, int q, const R s[6], R ep) {
+__device__ __forceinline__ void store_state(const Params &p, int e, int q, const R s[6], R ep,
+                                            bool ep_changed = true) {
   R *S = (R *)p.sig;
   R *P = (R *)p.epbar;
   const long ne = p.ne;
 #pragma unroll
   for (int c = 0; c < 6; c++) S[(long)(q * 6 + c) * ne + e] = s[c];
-  P[(long)q * ne + e] = ep;
+  if (ep_changed) P[(long)q * ne + e] = ep;
 }
 
 // State of one point loaded ahead of use (the tet4 workgroup kernel issues
-// it before its first barrier so the load overlaps the node staging).
+// it before its first barrier so the load overlaps the node staging);
+// point_stress leaves the updated ep in it.
 template <typename R> struct PreState {
   R s[6];
   R ep;
 };
+
+// The stress of one point, and its ep only when the point has yielded
+// before (ep of the others is 0).
+template <typename R>
+__device__ __forceinline__ void load_state_if(const Params &p, int e, int q, bool has_ep,
+                                              PreState<R> &st) {
+  const R *S = (const R *)p.sig;
+  const long ne = p.ne;
+#pragma unroll
+  for (int c = 0; c < 6; c++) st.s[c] = S[(long)(q * 6 + c) * ne + e];
+  st.ep = has_ep ? ((const R *)p.epbar)[(long)q * ne + e] : R(0);
+}
 
 // Stress at one point from the strain increment de.  compat: sigma = D de
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
@@ -80,7 +96,7 @@ template <typename R> struct PreState {
 template <typename R, int PHYS, bool PRE = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
-                                             const PreState<R> *pre = nullptr) {
+                                             PreState<R> *pre = nullptr) {
   const R c11 = R(p.c11), c12 = R(p.c12), c44 = R(p.c44);
   R ds[6];
   ds[0] = c11 * de[0] + c12 * (de[1] + de[2]);
@@ -122,7 +138,8 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-    store_state<R>(p, e, q, s, ep);
+    store_state<R>(p, e, q, s, ep, dg > R(0));
+    if constexpr (PRE) pre->ep = ep;
   }
 }
 
@@ -247,7 +264,7 @@ __device__ __forceinline__ void tet4_forces(const R Ji[9], const R s[6], R wd, c
 template <typename R, int PHYS, bool PRE, class Get, class Put>
 __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
                                           const Put &put, EAcc &ea,
-                                          const PreState<R> *pre = nullptr) {
+                                          PreState<R> *pre = nullptr) {
   R Ji[9], de[6], wd;
   tet4_kin<R, PHYS>(p, get, dt, Ji, de, wd, ea);
   R s[6];

@@ -287,10 +287,15 @@ k_elem_block4(Params p) {
   const int e = es + t;
   uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
   PreState<R> pre;
+  // yielded bits of this wave's elements
+  const long yi = (long)b * p.ywords + __builtin_amdgcn_readfirstlane(t >> 6);
+  unsigned long long yw = ~0ull;
+  if constexpr (PHYS != PHYS_COMPAT)
+    if (p.yielded) yw = p.yielded[yi];
   if (has_e) {
     lc4 = *(const uint2 *)(p.conn_local + 4L * e);
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
-    if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, 0, pre.s, pre.ep);
+    if constexpr (PHYS != PHYS_COMPAT) load_state_if<R>(p, e, 0, (yw >> (t & 63)) & 1ull, pre);
   }
   const bool has_n = g >= 0;
   if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
@@ -310,6 +315,10 @@ k_elem_block4(Params p) {
       for (int c = 0; c < 3; c++) dst[c] = fk[c];
     };
     tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+  }
+  if constexpr (PHYS != PHYS_COMPAT) {
+    const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
+    if (p.yielded && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
   __syncthreads();
 
@@ -409,7 +418,7 @@ k_elem_block4_mlp(Params p) {
 #pragma unroll
       for (int c = 0; c < 6; c++) wk += s[c] * de[c];
       ea.wint += double(wd * R(0.5) * wk);
-      store_state<R>(p, e, 0, s, pre.ep);
+      store_state<R>(p, e, 0, s, pre.ep, false);
       tet4_forces<R>(Ji, s, wd, put);
     } else {
       yielding = true;
