@@ -854,10 +854,15 @@ extern "C" int cpi_get_info(cpi_ctx *c, cpi_info *info) {
   info->num_blocks = c->scatter == CPI_SCATTER_BLOCK ? c->nblocks : 0;
   info->num_boundary_nodes = c->scatter == CPI_SCATTER_BLOCK ? c->plan.n_boundary : c->nn;
   info->num_partial_slots = c->scatter == CPI_SCATTER_BLOCK ? c->plan.n_slots : 0;
-  // algorithmic bytes: connectivity 4 npe and state 14 nqs s per element,
-  // then per node what each pass must touch once
+  // algorithmic bytes of an elastic step: connectivity 4 npe and the
+  // stress read and written, 12 nq s, per element (ep is written only when
+  // a point yields; the exact-return tet4 block kernel reads it only for
+  // elements that have yielded, the other kernels read it always), then
+  // per node what each pass must touch once
   const double s = c->prec == CPI_PREC_F32 ? 4 : 8;
-  const double state = c->phys == CPI_PHYSICS_COMPAT ? 0.0 : 14.0 * c->nq * s;
+  const bool ep_read = !(c->npe == 4 && c->scatter == CPI_SCATTER_BLOCK && c->phys == CPI_PHYSICS_EP);
+  const double state =
+      c->phys == CPI_PHYSICS_COMPAT ? 0.0 : (12.0 + (ep_read ? 1.0 : 0.0)) * c->nq * s;
   const double per_elem = 4.0 * c->npe + state;
   info->bytes_per_step = c->ne * per_elem + 224.0 * c->nn;
   if (c->scatter == CPI_SCATTER_BLOCK) {
