@@ -589,7 +589,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   P.hist_cap = 4096;
   P.mlp_tol = opt->mlp_tolerance;
   if (c->phys == 2) {
-    if ((rc = mlp_alloc(c->mlp, c->stream, c->err))) return rc;
+    if ((rc = mlp_alloc(c->mlp, c->nblocks, c->stream, c->err))) return rc;
     P.mlp = c->mlp.d_params;
     P.mlp_stats = c->mlp.d_stats;
   }
@@ -1090,8 +1090,14 @@ extern "C" int cpi_get_surrogate_stats(cpi_ctx *c, int64_t *accepted, int64_t *f
   if (!c) return CPI_ERR_INVALID;
   unsigned long long h[2] = {0, 0};
   if (c->mlp.d_stats) {
-    HIPCHK(c, hipMemcpyAsync(h, c->mlp.d_stats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> all(2 * (size_t)c->mlp.nstats);
+    HIPCHK(c, hipMemcpyAsync(all.data(), c->mlp.d_stats, sizeof(unsigned long long) * all.size(),
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < all.size(); i += 2) {
+      h[0] += all[i];
+      h[1] += all[i + 1];
+    }
   }
   if (accepted) *accepted = (int64_t)h[0];
   if (fallback) *fallback = (int64_t)h[1];
@@ -1101,7 +1107,8 @@ extern "C" int cpi_get_surrogate_stats(cpi_ctx *c, int64_t *accepted, int64_t *f
 extern "C" int cpi_reset_surrogate_stats(cpi_ctx *c) {
   if (!c) return CPI_ERR_INVALID;
   if (c->mlp.d_stats) {
-    HIPCHK(c, hipMemsetAsync(c->mlp.d_stats, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlp.d_stats, 0, 2 * sizeof(unsigned long long) * c->mlp.nstats,
+                             c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return CPI_OK;

@@ -105,7 +105,8 @@ struct Params {
   int elem_count;
   const MlpParams *mlp;  // null unless the surrogate is enabled
   double mlp_tol;        // fallback threshold on |f_yield| / sy0
-  unsigned long long *mlp_stats;  // [2] surrogate accepted / fell back (optional)
+  unsigned long long *mlp_stats;  // [nblocks][2] surrogate accepted / fell back
+                                  // per block of the element kernel (optional)
   float *samples;        // return-mapping sample buffer (optional)
   int *sample_count;
   int sample_cap;
@@ -115,6 +116,22 @@ struct Params {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+
+// Element blocks in chunks of kXcdChunk consecutive (Hilbert-ordered, so
+// neighbouring) blocks, the chunks dealt round-robin to the 8 XCDs: the
+// blocks of a chunk share nodes in one L2, and a hot region spreads over
+// every XCD instead of landing on the few whose contiguous range covers it.
+// The surrogate kernel uses it: its plastic zone at the wall costs several
+// times an elastic block.  Hardware workgroup hw runs on XCD hw & 7 as its
+// (hw >> 3)-th; blocks past the last whole round of chunks keep their
+// index.
+constexpr int kXcdChunk = 16;
+__device__ __forceinline__ int xcd_chunked(int hw, int n) {
+  const int m = n / (8 * kXcdChunk) * (8 * kXcdChunk);
+  if (hw >= m) return hw;
+  const int x = hw & 7, k = hw >> 3;
+  return ((k / kXcdChunk) * 8 + x) * kXcdChunk + k % kXcdChunk;
+}
 
 // hardware dispatches workgroups round-robin over the 8 XCDs; give each XCD
 // a contiguous range of logical blocks so neighbouring blocks, which share

@@ -362,7 +362,8 @@ k_elem_block4_mlp(Params p) {
   extern __shared__ double smem[];
   __shared__ uint16_t queue[NT];
   __shared__ int wcnt[NT / 64];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  __shared__ unsigned mcnt[2];  // network accepted / fell back
+  const int b = xcd_chunked(blockIdx.x, gridDim.x);
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   int g = -1, slot = -1, deg = 0;
   if (t < p.ml) {
@@ -391,6 +392,7 @@ k_elem_block4_mlp(Params p) {
   const bool has_n = g >= 0;
   if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
   if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
+  if (t == 0) mcnt[0] = mcnt[1] = 0u;
   __syncthreads();
 
   auto unpack4 = [](uint2 v, int o[4]) {
@@ -470,7 +472,7 @@ k_elem_block4_mlp(Params p) {
       const R q = yield_overshoot<R>(p, s, ep);
       R dg = R(0);
       if (act && q > R(0)) dg = mlp_apply<R>(p, y, q, ep, s, how);
-      mlp_count(p, how == MLP_ACCEPTED, how == MLP_FALLBACK);
+      mlp_count(mcnt, how == MLP_ACCEPTED, how == MLP_FALLBACK);
       if (act) {
 #pragma unroll
         for (int c = 0; c < 6; c++) wk += s[c] * de[c];
@@ -490,6 +492,10 @@ k_elem_block4_mlp(Params p) {
       }
     }
     __syncthreads();
+    if (t == 0 && p.mlp_stats && (mcnt[0] | mcnt[1])) {
+      p.mlp_stats[2L * b] += mcnt[0];
+      p.mlp_stats[2L * b + 1] += mcnt[1];
+    }
   }
 
   if (has_n) {

@@ -347,14 +347,16 @@ __global__ void __launch_bounds__(64) k_mlp_return_points(Params p, int n, const
 // per-context buffers
 // ---------------------------------------------------------------------------
 
-int mlp_alloc(MlpHost &m, hipStream_t s, std::string &err) {
+int mlp_alloc(MlpHost &m, int nstats, hipStream_t s, std::string &err) {
+  m.nstats = nstats > 0 ? nstats : 1;
+  const size_t sb = 2 * sizeof(unsigned long long) * (size_t)m.nstats;
   if (hipMalloc(&m.d_params, sizeof(MlpParams)) != hipSuccess ||
       hipMalloc(&m.d_weights, sizeof(MlpWeights)) != hipSuccess ||
-      hipMalloc(&m.d_stats, 2 * sizeof(unsigned long long)) != hipSuccess) {
+      hipMalloc(&m.d_stats, sb) != hipSuccess) {
     err = "hipMalloc for the surrogate failed";
     return CPI_ERR_ALLOC;
   }
-  if (hipMemsetAsync(m.d_stats, 0, 2 * sizeof(unsigned long long), s) != hipSuccess ||
+  if (hipMemsetAsync(m.d_stats, 0, sb, s) != hipSuccess ||
       hipMemsetAsync(m.d_weights, 0, sizeof(MlpWeights), s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     err = "hipMemset for the surrogate failed";

@@ -17,7 +17,8 @@ namespace cpi {
 struct MlpHost {
   MlpParams *d_params = nullptr;       // device copy of the inference view
   MlpWeights *d_weights = nullptr;     // installed weights
-  unsigned long long *d_stats = nullptr;  // [0] accepted, [1] fell back
+  unsigned long long *d_stats = nullptr;  // [nstats][2] accepted, fell back
+  int nstats = 0;                      // one pair per element block
   bool ready = false;                  // weights installed
   float *d_samples = nullptr;          // [cap][kSampleFloats]
   int *d_sample_count = nullptr;
@@ -27,7 +28,7 @@ struct MlpHost {
 void set_global_error(const std::string &s);
 const char *global_error();
 
-int mlp_alloc(MlpHost &m, hipStream_t s, std::string &err);
+int mlp_alloc(MlpHost &m, int nstats, hipStream_t s, std::string &err);
 void mlp_free(MlpHost &m);
 int mlp_samples_alloc(MlpHost &m, int cap, hipStream_t s, std::string &err);
 

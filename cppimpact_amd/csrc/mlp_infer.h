@@ -147,13 +147,13 @@ __device__ __forceinline__ R mlp_apply(const Params &p, const float y[kMlpOut], 
   return radial_return<R>(p, s, ep);
 }
 
-// accepted / fell-back counters of one wave
-__device__ __forceinline__ void mlp_count(const Params &p, bool accepted, bool fell_back) {
-  if (!p.mlp_stats) return;
+// accepted / fell-back counts of one wave, added to the workgroup's pair in
+// LDS (the workgroup adds the pair to its own global counters once)
+__device__ __forceinline__ void mlp_count(unsigned *cnt, bool accepted, bool fell_back) {
   const uint64_t a = __ballot(accepted), f = __ballot(fell_back);
   if ((threadIdx.x & 63) == 0 && (a | f)) {
-    atomicAdd(p.mlp_stats, (unsigned long long)__popcll(a));
-    atomicAdd(p.mlp_stats + 1, (unsigned long long)__popcll(f));
+    atomicAdd(cnt, (unsigned)__popcll(a));
+    atomicAdd(cnt + 1, (unsigned)__popcll(f));
   }
 }
 
