@@ -13,7 +13,10 @@ and energies, MLP surrogate of the return map with exact fallback; the
 network is trained first on samples of small-cube impacts at speeds uniform
 in [5, 30] m/s, --no-surrogate keeps the exact return).  The headline value
 is the mixed-precision run; the plain run, the speedup and the displacement
-and plastic-strain errors between them ride along.
+and plastic-strain errors between them ride along.  A supplementary pair of
+runs on the thin-walled cylinder (1,198,800 tet10, oblique 7 m/s impact;
+--no-c3 skips it) reports the tet10 step in FP64 and FP32 under
+"cylinder_c3".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -47,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_HEX, help="hexes per side (default 64)")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the supplementary thin-walled cylinder (tet10) runs")
+    ap.add_argument("--c3-steps", type=int, default=100)
     return ap.parse_args()
 
 
@@ -134,6 +140,63 @@ def run_mode(x, conn, args, precision, stress_update, dt, net=None):
     )
 
 
+def run_c3(args, precision, dt):
+    """The thin-walled cylinder: R 1 m, L 4 m, t 0.02 m, 360 x 185 x 3 cells
+    x 6 tet10 (1,198,800 elements), oblique 7 m/s impact 30 degrees from the
+    wall normal, J2 aluminium, 4-point rule, exact return."""
+    import math
+
+    import cppimpact_amd as ci
+    from cppimpact_amd import _lib as L
+
+    E = 68.9e9
+    x, conn = ci.cylinder_mesh(360, 185, 3, 1.0, 4.0, 0.02, origin=(0.0, 0.0, 1e-6),
+                               quadratic=True)
+    a = math.radians(30.0)
+    mat = ci.Elastoplastic(E, 2700.0, 0.33, 1.0, 700e6, 276e6)
+    wall = ci.Wall("Wall", 0.0, 10 * E * 0.02 / 3, None, 0, 1)
+    dyn = ci.Dynamics(ci.Mesh.from_arrays(x, conn), mat, wall, physics=L.PHYSICS_EP,
+                      precision=precision)
+    dyn.vel[:] = (7.0 * math.sin(a), 0.0, -7.0 * math.cos(a))
+    dyn.init(dt)
+    dyn.step(10)
+    dyn.synchronize()
+    t0 = time.perf_counter()
+    dyn.time_steps(args.c3_steps)
+    dyn.synchronize()
+    wall_s = time.perf_counter() - t0
+    prof = dyn.profile_kernels(5)
+    dyn.synchronize()
+    info = dyn.info()
+    xg, *_ = dyn.fetch()
+    dyn.close()
+    elem = [v[0] / v[1] for k, v in prof.items() if k.startswith("elem")][0]
+    n = conn.shape[0]
+    return dict(x0=x, x=xg, dt=info.dt, elements=n,
+                value=n * args.c3_steps / wall_s, ms_per_step=1e3 * wall_s / args.c3_steps,
+                elem_ms=elem, elem_gbs=info.elem_kernel_bytes / (elem * 1e-3) / 1e9)
+
+
+def c3_summary(args):
+    import numpy as np
+
+    from cppimpact_amd import _lib as L
+
+    f64 = run_c3(args, L.PREC_F64, 0.0)
+    f32 = run_c3(args, L.PREC_F32, f64["dt"])
+    disp = np.abs(f64["x"] - f64["x0"]).max()
+    derr = float(np.abs(f32["x"] - f64["x"]).max() / disp) if disp > 0 else 0.0
+    row = lambda r: {"value": round(r["value"], 1), "ms_per_step": round(r["ms_per_step"], 5),
+                     "elem_kernel_ms": round(r["elem_ms"], 5),
+                     "elem_algorithmic_gbs": round(r["elem_gbs"], 1)}
+    return {"workload": f"cylinder R 1 m, L 4 m, t 0.02 m, 360x185x3 cells x 6 tet10 "
+                        f"({f64['elements']} elements), oblique 7 m/s at 30 deg, 4-point rule, "
+                        f"J2 ep, dt=0.5*stable, {args.c3_steps} steps after 10",
+            "unit": "element-steps/s", "plain_fp64": row(f64), "fp32": row(f32),
+            "speedup_fp32_vs_fp64": round(f32["value"] / f64["value"], 4),
+            "disp_rel_err_fp32_vs_fp64": derr}
+
+
 def roofline(r):
     achieved = r["elem_bytes"] / (r["elem_ms"] * 1e-3) / 1e9
     traffic = None
@@ -216,6 +279,11 @@ def main():
     derr = float(np.abs(mixed["x"] - plain["x"]).max() / disp) if disp > 0 else 0.0
     epmax = float(np.abs(plain["ep"]).max())
     eperr = float(np.abs(mixed["ep"] - plain["ep"]).max() / epmax) if epmax > 0 else 0.0
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_summary(args)
+        print(f"bench: c3 cylinder fp64 {c3['plain_fp64']['value']:.4g}, fp32 "
+              f"{c3['fp32']['value']:.4g} el-steps/s", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(x, conn, plain["dt"], args.cpu_steps)
@@ -258,6 +326,7 @@ def main():
         "node_kernel_ms": round(mixed["node_ms"], 5),
         "blocks": mixed["blocks"], "boundary_nodes": mixed["boundary_nodes"],
         "epbar_max": mixed["epbar_max"],
+        "cylinder_c3": c3,
     }
     if net is not None:
         net.close()
