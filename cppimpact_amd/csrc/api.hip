@@ -142,7 +142,7 @@ int enqueue_step(cpi_ctx *c, bool aux) {
     e = launch_elem_block(p, c->rule, c->prec, c->phys, aux, c->nblocks, c->smem, c->stream);
     if (e == hipSuccess) {
       p.epart_base = c->n_elem_rows;
-      e = launch_node_bnd(p, aux, c->stream);
+      e = launch_node_bnd(p, c->prec, aux, c->stream);
     }
   } else {
     const bool atomic = c->scatter == CPI_SCATTER_ATOMIC;
@@ -935,7 +935,7 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
     HIPCHK(c, hipEventRecord(ev[1], c->stream));
     p.epart_base = c->n_elem_rows;
     if (e == hipSuccess)
-      e = c->scatter == CPI_SCATTER_BLOCK ? launch_node_bnd(p, false, c->stream)
+      e = c->scatter == CPI_SCATTER_BLOCK ? launch_node_bnd(p, c->prec, false, c->stream)
                                           : launch_node_all(p, false, c->stream);
     launches[1] += 1;
     HIPCHK(c, hipEventRecord(ev[2], c->stream));

@@ -17,7 +17,8 @@ hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool
 hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem);
 hipError_t launch_elem_direct(const Params &p, int rule, int prec, int phys, bool atomic,
                               hipStream_t s);
-hipError_t launch_node_bnd(const Params &p, bool aux, hipStream_t s);
+// prec: the element kernel's precision (its partials are stored in it)
+hipError_t launch_node_bnd(const Params &p, int prec, bool aux, hipStream_t s);
 hipError_t launch_node_all(const Params &p, bool aux, hipStream_t s);
 hipError_t launch_node_init(const Params &p, hipStream_t s);
 hipError_t launch_energy_reduce(const Params &p, hipStream_t s);

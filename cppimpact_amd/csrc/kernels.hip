@@ -145,6 +145,19 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
   ea.wgrav += m * wg;
 }
 
+// One block's partial force on a shared node, stored in the element
+// kernel's precision: FP64 records (32 B) for FP64 elements, float4 (16 B)
+// for FP32 ones, whose element forces carry FP32 rounding already.  The
+// boundary-node kernel sums them in FP64.
+template <typename R>
+__device__ __forceinline__ void store_partial(const Params &p, int slot, const double f[3]) {
+  if constexpr (std::is_same<R, float>::value)
+    reinterpret_cast<float4 *>(p.partial)[slot] = make_float4(float(f[0]), float(f[1]),
+                                                              float(f[2]), 0.f);
+  else
+    p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+}
+
 // A workgroup's LDS node records, structure of arrays of ml doubles each:
 // x, y, z, m, vx, vy, vz.  Lanes gathering random nodes read 8-byte words at
 // an 8-byte stride, so the reads spread over the LDS banks (records of 64
@@ -248,7 +261,7 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
       ln.get(j, x, v);
       node_update<AUX>(p, g, f, x, v, ln.mass(j), dt, ea);
     } else {
-      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+      store_partial<R>(p, slot, f);
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
@@ -335,7 +348,7 @@ k_elem_block4(Params p) {
       ln.get(t, x, v);
       node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
     } else {
-      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+      store_partial<R>(p, slot, f);
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
@@ -511,14 +524,16 @@ k_elem_block4_mlp(Params p) {
       ln.get(t, x, v);
       node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
     } else {
-      p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+      store_partial<R>(p, slot, f);
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
-template <bool AUX>
+template <bool AUX, typename R>
 __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
+  using PR = typename std::conditional<std::is_same<R, float>::value, float4, Node4>::type;
+  const PR *part = reinterpret_cast<const PR *>(p.partial);
   const int i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * 4);
   const double dt = p.ctl->dt;
@@ -530,26 +545,25 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
     double f[3] = {0.0, 0.0, 0.0};
     const int s1 = p.slot_ptr[i + 1];
     int s = p.slot_ptr[i];
-    auto add = [&](const Node4 &q) {
-      f[0] += q.x;
-      f[1] += q.y;
-      f[2] += q.z;
+    auto add = [&](const PR &q) {
+      f[0] += double(q.x);
+      f[1] += double(q.y);
+      f[2] += double(q.z);
     };
     for (; s + 4 <= s1; s += 4) {
-      const Node4 q0 = p.partial[s], q1 = p.partial[s + 1], q2 = p.partial[s + 2],
-                  q3 = p.partial[s + 3];
+      const PR q0 = part[s], q1 = part[s + 1], q2 = part[s + 2], q3 = part[s + 3];
       add(q0);
       add(q1);
       add(q2);
       add(q3);
     }
     if (s + 2 <= s1) {
-      const Node4 q0 = p.partial[s], q1 = p.partial[s + 1];
+      const PR q0 = part[s], q1 = part[s + 1];
       add(q0);
       add(q1);
       s += 2;
     }
-    if (s < s1) add(p.partial[s]);
+    if (s < s1) add(part[s]);
     const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
@@ -1166,10 +1180,14 @@ hipError_t launch_elem_direct(const Params &p, int rule, int prec, int phys, boo
   return dispatch<ElemDirectF>(p.npe, rule, prec, phys, p, atomic, s);
 }
 
-hipError_t launch_node_bnd(const Params &p, bool aux, hipStream_t s) {
+hipError_t launch_node_bnd(const Params &p, int prec, bool aux, hipStream_t s) {
   const int grid = std::max(1, (p.n_boundary + 255) / 256);
-  if (aux) CPI_LAUNCH(k_node_bnd<true>, grid, 256, 0, s, p);
-  CPI_LAUNCH(k_node_bnd<false>, grid, 256, 0, s, p);
+  if (prec == 1) {
+    if (aux) CPI_LAUNCH((k_node_bnd<true, float>), grid, 256, 0, s, p);
+    CPI_LAUNCH((k_node_bnd<false, float>), grid, 256, 0, s, p);
+  }
+  if (aux) CPI_LAUNCH((k_node_bnd<true, double>), grid, 256, 0, s, p);
+  CPI_LAUNCH((k_node_bnd<false, double>), grid, 256, 0, s, p);
 }
 
 hipError_t launch_node_all(const Params &p, bool aux, hipStream_t s) {
