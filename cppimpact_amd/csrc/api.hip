@@ -46,6 +46,8 @@ struct cpi_ctx {
   std::vector<int> color_rows;  // first energy row of each colour launch
   size_t smem = 0;
   hipGraphExec_t g_plain = nullptr, g_aux = nullptr;
+  // kGraphRuns[k] plain steps captured in one graph each
+  hipGraphExec_t g_plain_run[2] = {nullptr, nullptr};
   bool mass_ready = false, inited = false;
   double dt = 0.0;
   int64_t step = 0;
@@ -163,10 +165,16 @@ int enqueue_step(cpi_ctx *c, bool aux) {
   return CPI_OK;
 }
 
-int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out) {
+// plain steps captured back to back in g_plain_run: one graph launch, and
+// no gap for the launch between the steps' kernels, per kGraphRuns[k]
+// steps (measured at c2: ≈ 4 µs less per step than one graph per step)
+constexpr int kGraphRuns[2] = {32, 4};
+
+int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out, int nrep = 1) {
   hipGraph_t g = nullptr;
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(c, aux);
+  int rc = CPI_OK;
+  for (int i = 0; i < nrep && rc == CPI_OK; i++) rc = enqueue_step(c, aux);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
@@ -182,6 +190,10 @@ int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out) {
 void drop_graphs(cpi_ctx *c) {
   if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
   if (c->g_aux) (void)hipGraphExecDestroy(c->g_aux);
+  for (auto &g : c->g_plain_run) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
   c->g_plain = c->g_aux = nullptr;
 }
 
@@ -690,7 +702,10 @@ extern "C" int cpi_init(cpi_ctx *c, double dt) {
   HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->opt.use_graph) {
-    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux))) return rc;
+    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux)) ||
+        (rc = capture(c, false, &c->g_plain_run[0], kGraphRuns[0])) ||
+        (rc = capture(c, false, &c->g_plain_run[1], kGraphRuns[1])))
+      return rc;
   }
   c->inited = true;
   return CPI_OK;
@@ -700,14 +715,22 @@ extern "C" int cpi_step(cpi_ctx *c, int32_t nsteps) {
   if (!c) return CPI_ERR_INVALID;
   if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_step before cpi_init");
   if (nsteps < 0) return fail(c, CPI_ERR_INVALID, "nsteps < 0");
-  for (int i = 0; i < nsteps; i++) {
+  for (int i = 0; i < nsteps;) {
     const bool aux = (i == nsteps - 1);
     if (c->g_plain) {
+      int k = 0;
+      while (k < 2 && !(c->g_plain_run[k] && nsteps - 1 - i >= kGraphRuns[k])) k++;
+      if (!aux && k < 2) {
+        HIPCHK(c, hipGraphLaunch(c->g_plain_run[k], c->stream));
+        i += kGraphRuns[k];
+        continue;
+      }
       HIPCHK(c, hipGraphLaunch(aux ? c->g_aux : c->g_plain, c->stream));
     } else {
       int rc = enqueue_step(c, aux);
       if (rc) return rc;
     }
+    i++;
   }
   c->step += nsteps;
   return CPI_OK;
@@ -1040,7 +1063,10 @@ int refresh_graphs(cpi_ctx *c) {
   drop_graphs(c);
   if (c->inited && c->opt.use_graph) {
     int rc;
-    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux))) return rc;
+    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux)) ||
+        (rc = capture(c, false, &c->g_plain_run[0], kGraphRuns[0])) ||
+        (rc = capture(c, false, &c->g_plain_run[1], kGraphRuns[1])))
+      return rc;
   }
   return CPI_OK;
 }
