@@ -16,7 +16,9 @@ is the mixed-precision run; the plain run, the speedup and the displacement
 and plastic-strain errors between them ride along.  A supplementary pair of
 runs on the thin-walled cylinder (1,198,800 tet10, oblique 7 m/s impact;
 --no-c3 skips it) reports the tet10 step in FP64 and FP32 under
-"cylinder_c3".
+"cylinder_c3".  Two CPU lines: "cpu_baseline" is the reference algorithm
+on one core, "cpu_baseline_threaded" the FP64 elastoplastic step on up to
+16 threads.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -49,6 +51,10 @@ def parse():
     ap.add_argument("--train-epochs", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-ep-steps", type=int, default=40,
+                    help="steps of the threaded FP64 elastoplastic CPU line")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of that line (default: min(16, cores))")
     ap.add_argument("--n", type=int, default=N_HEX, help="hexes per side (default 64)")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the supplementary thin-walled cylinder (tet10) runs")
@@ -231,6 +237,23 @@ def cpu_baseline(x, conn, dt, nsteps):
             "host_cpu": _cpu_model(), "host_nproc": os.cpu_count()}
 
 
+def cpu_baseline_threaded(x, conn, dt, nsteps, threads, speed):
+    """The oracle's FP64 elastoplastic step (the same physics as the plain
+    FP64 GPU run) with its element and node loops on several threads."""
+    from oracle import pyoracle as O
+    E = 68.9e9
+    wall = dict(location=0.0, stiffness=10 * E * SIDE / N_HEX)
+    s = O.Solver(x, conn, (0.0, 0.0, -speed), E=E, nu=0.33, rho=2700.0, sigma_y=276e6,
+                 H=700e6, physics=O.EP, wall=wall, dt=dt)
+    out = s.solve(nsteps, with_energy=False, threads=threads)
+    secs = out["loop_seconds"]
+    return {"value": round(conn.shape[0] * nsteps / secs, 1), "unit": "element-steps/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{nsteps} loop iteration(s) of the FP64 elastoplastic step (J2 radial "
+                      f"return, centroid rule, HRZ mass) over the full {conn.shape[0]}-element "
+                      f"mesh on {threads} threads, initialisation excluded ({secs:.1f} s)"}
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -284,10 +307,14 @@ def main():
         c3 = c3_summary(args)
         print(f"bench: c3 cylinder fp64 {c3['plain_fp64']['value']:.4g}, fp32 "
               f"{c3['fp32']['value']:.4g} el-steps/s", file=sys.stderr, flush=True)
-    cpu = None
+    cpu, cpu_mt = None, None
     if not args.no_cpu:
         cpu = cpu_baseline(x, conn, plain["dt"], args.cpu_steps)
         print(f"bench: cpu baseline {cpu['value']:.4g} el-steps/s", file=sys.stderr, flush=True)
+        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu_mt = cpu_baseline_threaded(x, conn, plain["dt"], args.cpu_ep_steps, nth, args.speed)
+        print(f"bench: cpu ep baseline, {nth} threads: {cpu_mt['value']:.4g} el-steps/s",
+              file=sys.stderr, flush=True)
     line = {
         "metric": "element-steps/s, 1 MI355X: FP32+surrogate vs FP64 plain at <=1e-3 disp err",
         "value": round(mixed["value"], 1),
@@ -312,6 +339,7 @@ def main():
         },
         "roofline": roofline(mixed),
         "cpu_baseline": cpu,
+        "cpu_baseline_threaded": cpu_mt,
         "plain_fp64": {"value": round(plain["value"], 1),
                        "ms_per_step": round(plain["ms_per_step"], 5),
                        "roofline": roofline(plain)},

@@ -14,6 +14,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -562,6 +563,24 @@ double orc_stable_dt(const orc_problem *p, double *dt_contact) {
   return dte;
 }
 
+/* The node loop of update() for node i (dynamics_kernels.h:150-168): wall
+   penalty at x + du, then gravity; a held node keeps zero acceleration. */
+static inline void node_external(const orc_problem *p, orc_state *s,
+                                 const double *du, int i) {
+  const int dim = p->wall_dim;
+  const double nk = p->wall_normal * p->wall_stiffness;
+  const int held = p->fixed_mask && p->fixed_mask[i];
+  if (p->wall_enabled && !held && (!p->contact_mask || p->contact_mask[i])) {
+    double pos = s->x[3 * i + dim] + du[3 * i + dim];
+    double dist = (pos - p->wall_location) * p->wall_normal;
+    if (dist < 0.0) s->a[3 * i + dim] += -1 * (1 / s->m[i]) * dist * nk;
+  }
+  s->a[3 * i + 0] += p->gravity[0];
+  s->a[3 * i + 1] += p->gravity[1];
+  s->a[3 * i + 2] += p->gravity[2];
+  if (held) s->a[3 * i + 0] = s->a[3 * i + 1] = s->a[3 * i + 2] = 0.0;
+}
+
 /* One update() call (dynamics_kernels.h:8-169 for compat): zero the
    acceleration, assemble the mass on the first call, element loop with
    per-element -f_e/m accumulation, then the node loop with the wall penalty
@@ -633,20 +652,7 @@ static void update(const orc_problem *p, orc_state *s, const double *du,
       for (int k = 0; k < 3; k++) s->a[3 * i + k] = -s->a[3 * i + k] / s->m[i];
   }
   *mindt = md;
-  const int dim = p->wall_dim;
-  const double nk = p->wall_normal * p->wall_stiffness;
-  for (int i = 0; i < nn; i++) {
-    const int held = p->fixed_mask && p->fixed_mask[i];
-    if (p->wall_enabled && !held && (!p->contact_mask || p->contact_mask[i])) {
-      double pos = s->x[3 * i + dim] + du[3 * i + dim];
-      double dist = (pos - p->wall_location) * p->wall_normal;
-      if (dist < 0.0) s->a[3 * i + dim] += -1 * (1 / s->m[i]) * dist * nk;
-    }
-    s->a[3 * i + 0] += p->gravity[0];
-    s->a[3 * i + 1] += p->gravity[1];
-    s->a[3 * i + 2] += p->gravity[2];
-    if (held) s->a[3 * i + 0] = s->a[3 * i + 1] = s->a[3 * i + 2] = 0.0;
-  }
+  for (int i = 0; i < nn; i++) node_external(p, s, du, i);
 }
 
 static void energy_row(const orc_problem *p, const orc_state *s,
@@ -676,15 +682,216 @@ static double now_seconds(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
-int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
-                    double *loop_seconds);
+/* ---------------------------------------------------------------------- */
+/* threaded step (same arithmetic, same summation order)                   */
+/* ---------------------------------------------------------------------- */
+
+/* The threaded update splits the serial one in two passes: each element's
+   nodal contributions and work go to per-element buffers (any order), then
+   each node sums its contributions in ascending element order, the order in
+   which the serial element loop adds them.  The work terms are summed in
+   element order on one thread.  Every floating-point operation is the
+   serial one, so the results are bit-identical to orc_solve. */
+#define ORC_MAX_THREADS 64
+
+typedef struct par_ctx par_ctx;
+typedef void (*range_fn)(par_ctx *, int lo, int hi);
+struct par_ctx {
+  const orc_problem *p;
+  orc_state *s;
+  const double *du;
+  double *fbuf;      /* [E][npe][3] -f_e/m (compat) or f_e (ep) */
+  double *wbuf;      /* [E][2] internal / plastic work of the element */
+  double *dtbuf;     /* [E] element stable dt (ep) */
+  int *nstart;       /* [N+1] node -> first slot */
+  int *nslot;        /* [E*npe] slots e*npe+j in ascending e */
+  double dt;
+  int nthreads;
+  range_fn fn;
+};
+
+typedef struct {
+  par_ctx *c;
+  int lo, hi;
+} par_job;
+
+static void *par_run(void *a) {
+  par_job *j = (par_job *)a;
+  j->c->fn(j->c, j->lo, j->hi);
+  return NULL;
+}
+
+/* fn over [0, n) in nthreads static chunks; chunk 0 (and any chunk whose
+   thread cannot be started) runs on the calling thread */
+static void par_for(par_ctx *c, range_fn fn, int n) {
+  const int nt = c->nthreads;
+  pthread_t th[ORC_MAX_THREADS];
+  par_job jb[ORC_MAX_THREADS];
+  int started[ORC_MAX_THREADS];
+  c->fn = fn;
+  for (int t = 0; t < nt; t++) {
+    jb[t].c = c;
+    jb[t].lo = (int)((long)n * t / nt);
+    jb[t].hi = (int)((long)n * (t + 1) / nt);
+    started[t] = t > 0 && pthread_create(&th[t], NULL, par_run, &jb[t]) == 0;
+  }
+  for (int t = 0; t < nt; t++)
+    if (!started[t]) par_run(&jb[t]);
+  for (int t = 1; t < nt; t++)
+    if (started[t]) pthread_join(th[t], NULL);
+}
+
+/* the element loop of update() for elements [lo, hi) */
+static void par_elements(par_ctx *c, int lo, int hi) {
+  const orc_problem *p = c->p;
+  const orc_state *s = c->s;
+  const int npe = p->npe, nd = 3 * npe;
+  const int nq = orc_quad_npts(p->quad_rule);
+  double xe[30], due[30];
+  for (int e = lo; e < hi; e++) {
+    const int *cn = p->conn + (size_t)npe * e;
+    double *fe = c->fbuf + (size_t)nd * e;
+    gather(npe, cn, s->x, xe);
+    gather(npe, cn, c->du, due);
+    if (p->physics == ORC_COMPAT) {
+      double minv[30];
+      for (int j = 0; j < npe; j++)
+        for (int k = 0; k < 3; k++) minv[3 * j + k] = 1.0 / s->m[cn[j]];
+      orc_element_force_compat(npe, p->quad_rule, p->E, p->nu, xe, due, fe);
+      double dw = 0.0;
+      for (int j = 0; j < nd; j++) dw += fe[j] * due[j];
+      c->wbuf[2 * (size_t)e] = dw;
+      c->wbuf[2 * (size_t)e + 1] = 0.0;
+      for (int j = 0; j < nd; j++) fe[j] = minv[j] * (-fe[j]);
+    } else {
+      double w2[2] = {0.0, 0.0};
+      c->dtbuf[e] = orc_element_dt(npe, p->E, p->nu, p->rho, xe);
+      orc_element_force_ep(npe, p->quad_rule, p->E, p->nu, p->sigma_y, p->H,
+                           xe, due, s->sig + (size_t)6 * nq * e,
+                           s->epbar + (size_t)nq * e, fe, w2);
+      c->wbuf[2 * (size_t)e] = w2[0];
+      c->wbuf[2 * (size_t)e + 1] = w2[1];
+    }
+  }
+}
+
+/* the node side of update() for nodes [lo, hi) */
+static void par_nodes(par_ctx *c, int lo, int hi) {
+  const orc_problem *p = c->p;
+  orc_state *s = c->s;
+  for (int i = lo; i < hi; i++) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int q = c->nstart[i]; q < c->nstart[i + 1]; q++) {
+      const double *f = c->fbuf + 3 * (size_t)c->nslot[q];
+      a0 += f[0];
+      a1 += f[1];
+      a2 += f[2];
+    }
+    s->a[3 * i + 0] = a0;
+    s->a[3 * i + 1] = a1;
+    s->a[3 * i + 2] = a2;
+    if (p->physics != ORC_COMPAT)
+      for (int k = 0; k < 3; k++) s->a[3 * i + k] = -s->a[3 * i + k] / s->m[i];
+    node_external(p, s, c->du, i);
+  }
+}
+
+/* the loop body's nodal update (x, v, v_i) for dofs [lo, hi) */
+static void par_kick(par_ctx *c, int lo, int hi) {
+  orc_state *s = c->s;
+  const double *du = c->du;
+  const double dt = c->dt;
+  for (int i = lo; i < hi; i++) {
+    s->x[i] += du[i];
+    s->v[i] += dt * s->a[i];
+    s->v_i[i] = s->v[i] - 0.5 * dt * s->a[i];
+  }
+}
+
+static void update_par(par_ctx *c, const double *du, double *wint, double *wpl,
+                       double *mindt) {
+  const orc_problem *p = c->p;
+  const int ne = p->num_elements;
+  c->du = du;
+  par_for(c, par_elements, ne);
+  double md = INFINITY;
+  for (int e = 0; e < ne; e++) {
+    *wint += c->wbuf[2 * (size_t)e];
+    if (p->physics != ORC_COMPAT) {
+      *wpl += c->wbuf[2 * (size_t)e + 1];
+      if (c->dtbuf[e] < md) md = c->dtbuf[e];
+    }
+  }
+  *mindt = md;
+  par_for(c, par_nodes, p->num_nodes);
+}
+
+static void par_free(par_ctx *c) {
+  free(c->fbuf);
+  free(c->wbuf);
+  free(c->dtbuf);
+  free(c->nstart);
+  free(c->nslot);
+}
+
+/* buffers and the node -> slot lists; 0 on success */
+static int par_setup(par_ctx *c, const orc_problem *p, orc_state *s,
+                     int nthreads) {
+  const int nn = p->num_nodes, ne = p->num_elements, npe = p->npe;
+  const size_t ns = (size_t)ne * npe;
+  memset(c, 0, sizeof(*c));
+  c->p = p;
+  c->s = s;
+  c->dt = p->dt;
+  c->nthreads = nthreads > ORC_MAX_THREADS ? ORC_MAX_THREADS : nthreads;
+  c->fbuf = (double *)malloc(sizeof(double) * 3 * (ns ? ns : 1));
+  c->wbuf = (double *)malloc(sizeof(double) * 2 * (ne ? ne : 1));
+  c->dtbuf = (double *)malloc(sizeof(double) * (ne ? ne : 1));
+  c->nstart = (int *)calloc((size_t)nn + 1, sizeof(int));
+  c->nslot = (int *)malloc(sizeof(int) * (ns ? ns : 1));
+  if (!c->fbuf || !c->wbuf || !c->dtbuf || !c->nstart || !c->nslot) {
+    par_free(c);
+    return -1;
+  }
+  for (size_t q = 0; q < ns; q++) c->nstart[p->conn[q] + 1]++;
+  for (int i = 0; i < nn; i++) c->nstart[i + 1] += c->nstart[i];
+  int *fill = (int *)malloc(sizeof(int) * (nn ? nn : 1));
+  if (!fill) {
+    par_free(c);
+    return -1;
+  }
+  memcpy(fill, c->nstart, sizeof(int) * nn);
+  for (size_t q = 0; q < ns; q++) c->nslot[fill[p->conn[q]]++] = (int)q;
+  free(fill);
+  return 0;
+}
+
+static int solve_impl(const orc_problem *p, int nsteps, orc_state *s,
+                      double *loop_seconds, par_ctx *pc);
 
 int orc_solve(const orc_problem *p, int nsteps, orc_state *s) {
-  return orc_solve_timed(p, nsteps, s, NULL);
+  return solve_impl(p, nsteps, s, NULL, NULL);
 }
 
 int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
                     double *loop_seconds) {
+  return solve_impl(p, nsteps, s, loop_seconds, NULL);
+}
+
+int orc_solve_threaded(const orc_problem *p, int nsteps, orc_state *s,
+                       int nthreads, double *loop_seconds) {
+  /* the last-writer-wins strain/stress output follows the serial loop */
+  if (nthreads <= 1 || (s->strain && s->stress))
+    return solve_impl(p, nsteps, s, loop_seconds, NULL);
+  par_ctx c;
+  if (par_setup(&c, p, s, nthreads) != 0) return -1;
+  int rc = solve_impl(p, nsteps, s, loop_seconds, &c);
+  par_free(&c);
+  return rc;
+}
+
+static int solve_impl(const orc_problem *p, int nsteps, orc_state *s,
+                      double *loop_seconds, par_ctx *pc) {
   const int nn = p->num_nodes, nd = 3 * nn;
   const int nq = orc_quad_npts(p->quad_rule);
   const double dt = p->dt;
@@ -720,11 +927,16 @@ int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
   double t0 = now_seconds();
   for (int n = 1; n <= nsteps; n++) {
     for (int j = 0; j < nd; j++) du[j] = dt * s->v[j];
-    update(p, s, du, &wint, &wpl, &mindt);
-    for (int i = 0; i < nd; i++) {
-      s->x[i] += du[i];
-      s->v[i] += dt * s->a[i];
-      s->v_i[i] = s->v[i] - 0.5 * dt * s->a[i];
+    if (pc) {
+      update_par(pc, du, &wint, &wpl, &mindt);
+      par_for(pc, par_kick, nd);
+    } else {
+      update(p, s, du, &wint, &wpl, &mindt);
+      for (int i = 0; i < nd; i++) {
+        s->x[i] += du[i];
+        s->v[i] += dt * s->a[i];
+        s->v_i[i] = s->v[i] - 0.5 * dt * s->a[i];
+      }
     }
     if (s->energy) {
       for (int i = 0; i < nn; i++)

@@ -156,6 +156,15 @@ int orc_solve(const orc_problem *p, int nsteps, orc_state *s);
 int orc_solve_timed(const orc_problem *p, int nsteps, orc_state *s,
                     double *loop_seconds);
 
+/* orc_solve_timed with the element and node loops split over nthreads
+   POSIX threads.  Elements write their nodal contributions to per-element
+   buffers and each node sums them in ascending element order, so the
+   results are bit-identical to orc_solve for any thread count.  Runs the
+   serial loop when nthreads <= 1 or strain/stress output is requested.
+   Returns -1 on allocation failure. */
+int orc_solve_threaded(const orc_problem *p, int nsteps, orc_state *s,
+                       int nthreads, double *loop_seconds);
+
 /* Stable time step of the mesh at X0 (min over elements), and the contact
    spring limit 2*sqrt(m_min/k) when k > 0 (returned in *dt_contact). */
 double orc_stable_dt(const orc_problem *p, double *dt_contact);

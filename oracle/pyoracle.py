@@ -79,6 +79,9 @@ def lib():
         L.orc_stable_dt.argtypes = [C.POINTER(Problem), _dp]
         L.orc_solve_timed.restype = C.c_int
         L.orc_solve_timed.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(State), _dp]
+        L.orc_solve_threaded.restype = C.c_int
+        L.orc_solve_threaded.argtypes = [C.POINTER(Problem), C.c_int, C.POINTER(State), C.c_int,
+                                         _dp]
         _LIB = L
     return _LIB
 
@@ -254,7 +257,8 @@ class Solver:
         de = lib().orc_stable_dt(C.byref(self.p), C.byref(dc))
         return de, dc.value
 
-    def solve(self, nsteps, with_strain=False, with_energy=True, timed=False):
+    def solve(self, nsteps, with_strain=False, with_energy=True, timed=False, threads=1):
+        """threads > 1 runs orc_solve_threaded (bit-identical results)."""
         n, e = self.p.num_nodes, self.p.num_elements
         out = {k: np.zeros((n, 3)) for k in ("x", "v", "a", "v_i")}
         out["m"] = np.zeros(n)
@@ -272,7 +276,11 @@ class Solver:
         for k in ("strain", "stress", "energy"):
             setattr(s, k, _d(out[k]) if k in out else None)
         secs = C.c_double(0.0)
-        rc = lib().orc_solve_timed(C.byref(self.p), nsteps, C.byref(s), C.byref(secs))
+        if threads > 1:
+            rc = lib().orc_solve_threaded(C.byref(self.p), nsteps, C.byref(s), int(threads),
+                                          C.byref(secs))
+        else:
+            rc = lib().orc_solve_timed(C.byref(self.p), nsteps, C.byref(s), C.byref(secs))
         if rc != 0:
             raise RuntimeError("orc_solve failed")
         out["loop_seconds"] = secs.value
