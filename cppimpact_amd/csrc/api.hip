@@ -405,11 +405,16 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   // device orders
   if (c->scatter == CPI_SCATTER_BLOCK) {
-    const int nt = elem_block_threads(npe);
-    int me = opt->block_elements > 0 ? opt->block_elements : nt;
-    // the tet4 kernel gives each thread one element and at most one node
+    const int nt = elem_block_threads(npe, c->rule);
+    const int dflt = elem_block_default_elements(npe);
+    int me = opt->block_elements > 0 ? opt->block_elements : dflt;
+    // the tet4 kernel gives each thread one element and at most one node,
+    // the tet10 kernel of the 4-point rules each thread one point
     if (npe == 4 && me > nt)
       return fail(c, CPI_ERR_INVALID, "block_elements above " + std::to_string(nt) + " for tet4");
+    if (npe == 10 && c->nq == 4 && 4 * me > nt)
+      return fail(c, CPI_ERR_INVALID, "block_elements above " + std::to_string(nt / 4) +
+                                          " for tet10 on a 4-point rule");
     // LDS budget per workgroup: three (FP64) or four (FP32) tet4 workgroups,
     // two tet10 ones per 160 KiB CU, never above the device's limit; node
     // records take 56 bytes each (slot offsets budgeted for 64 contributions
@@ -516,8 +521,10 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   }
   if (c->scatter == CPI_SCATTER_BLOCK) {
     BlockPlan &B = c->plan;
-    if (npe == 4 && B.max_local_nodes > elem_block_threads(4))
+    if (npe == 4 && B.max_local_nodes > elem_block_threads(4, c->rule))
       return fail(c, CPI_ERR_INVALID, "partition: a tet4 block has more nodes than threads");
+    if (npe == 10 && c->nq == 4 && 4 * B.max_block_elems > elem_block_threads(10, c->rule))
+      return fail(c, CPI_ERR_INVALID, "partition: a tet10 block has more points than threads");
     int32_t *meta, *bn, *bs, *sp, *dn, *ds;
     uint16_t *cl, *ci, *dg, *dof;
     Node4 *part;
@@ -568,7 +575,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       return fail(c, CPI_ERR_HIP, std::string("LDS opt-in for the element kernel: ") +
                                       hipGetErrorString(pe));
     // one energy row per wave
-    c->n_elem_rows = c->nblocks * (elem_block_threads(npe) / 64);
+    c->n_elem_rows = c->nblocks * (elem_block_threads(npe, c->rule) / 64);
     c->n_node_rows = 4 * std::max(1, (B.n_boundary + 255) / 256);
   } else {
     const int nt = elem_direct_threads(npe);

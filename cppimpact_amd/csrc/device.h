@@ -142,6 +142,20 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
   return xcd * q + min(xcd, r) + k;
 }
 
+// lane K's value of each group of four lanes (4k..4k+3), on all four lanes
+// (DPP quad_perm broadcast; the source lane must be active)
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, K * 0x55, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), K * 0x55, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 template <typename R>
 __device__ __forceinline__ R inv3(const R J[9], R Ji[9]) {
   R c0 = J[4] * J[8] - J[5] * J[7];

@@ -164,12 +164,21 @@ __device__ __forceinline__ R ep_trial(const Params &p, const R de[6], const R so
   return w;
 }
 
-// f_i += wd * sigma . dN_i
+// sigma . dN_i (engineering-shear Voigt order xx, yy, zz, xy, yz, xz)
+template <typename R>
+__device__ __forceinline__ void sig_dn(const R dn[3], const R s[6], R g[3]) {
+  g[0] = dn[0] * s[0] + dn[1] * s[3] + dn[2] * s[5];
+  g[1] = dn[1] * s[1] + dn[0] * s[3] + dn[2] * s[4];
+  g[2] = dn[2] * s[2] + dn[1] * s[4] + dn[0] * s[5];
+}
+
+// f_i += wd * sigma . dN_i, one rounding per component (fused)
 template <typename R>
 __device__ __forceinline__ void add_bts(R f[3], const R dn[3], const R s[6], R wd) {
-  f[0] += wd * (dn[0] * s[0] + dn[1] * s[3] + dn[2] * s[5]);
-  f[1] += wd * (dn[1] * s[1] + dn[0] * s[3] + dn[2] * s[4]);
-  f[2] += wd * (dn[2] * s[2] + dn[1] * s[4] + dn[0] * s[5]);
+  R g[3];
+  sig_dn<R>(dn, s, g);
+#pragma unroll
+  for (int c = 0; c < 3; c++) f[c] = fma(wd, g[c], f[c]);
 }
 
 // minimum altitude of the corner tetrahedron / wave speed
@@ -342,6 +351,100 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
   }
   if constexpr (PHYS != PHYS_COMPAT) {
     // corners in the linear ordering: (2, 0, 1, 3) is an even permutation
+    R e1[3], e2[3], e3[3];
+    double xa[3], va[3];
+    get(0, xa, va);
+#pragma unroll
+    for (int c = 0; c < 3; c++) e1[c] = R(xa[c] - o[c]);
+    get(1, xa, va);
+#pragma unroll
+    for (int c = 0; c < 3; c++) e2[c] = R(xa[c] - o[c]);
+    get(3, xa, va);
+#pragma unroll
+    for (int c = 0; c < 3; c++) e3[c] = R(xa[c] - o[c]);
+    R v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+           e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+    ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, e3, v6, p.wave_speed, kTet10LengthFactor));
+  }
+}
+
+// tet10 on a 4-point rule with one lane per point: lanes 4k..4k+3 hold the
+// same element e, lane q = (lane & 3) point q.  The point's kinematics and
+// stress are the arithmetic of tet10_points for that point; each corner
+// force is then summed over the quad's lanes in rule order, fused the way
+// add() accumulates it point by point from zero (DPP broadcasts inside the
+// quad), so every lane holds corner i's element force, bit for bit the one
+// of the point loop, and hands it to put(i, f).  All four lanes of a quad
+// must be active.  pre: the point's stress state, loaded ahead (EP models).
+template <int RULE, typename R, int PHYS, class Get, class Put>
+__device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
+                                                 int e, int q, PreState<R> &pre, const Put &put,
+                                                 EAcc &ea) {
+  using Q = Quad<RULE>;
+  static_assert(Q::n == 4, "one lane per point of a 4-point rule");
+  double o[3];
+  {
+    double ov[3];
+    get(2, o, ov);
+  }
+  double Gd[30];
+  tet10_grad(Q::pt(q, 0), Q::pt(q, 1), Q::pt(q, 2), Gd);
+  R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    double xi[3], vi[3];
+    get(i, xi, vi);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const R d = R(xi[a] - o[a]);
+#pragma unroll
+      for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * d;
+    }
+  }
+  R Ji[9];
+  R detJ = inv3<R>(J, Ji);
+  if (!(detJ > R(0))) ea.bad += 1.0;
+  R de[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    double xi[3], vi[3];
+    get(i, xi, vi);
+    R u[3] = {R(dt * vi[0]), R(dt * vi[1]), R(dt * vi[2])};
+    R dn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
+              Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+    de[0] += dn[0] * u[0];
+    de[1] += dn[1] * u[1];
+    de[2] += dn[2] * u[2];
+    de[3] += dn[1] * u[0] + dn[0] * u[1];
+    de[4] += dn[2] * u[1] + dn[1] * u[2];
+    de[5] += dn[2] * u[0] + dn[0] * u[2];
+  }
+  const R wd = R(Q::w(q)) * detJ;
+  R s[6];
+  point_stress<R, PHYS, true>(p, e, q, de, wd, s, ea, &pre);
+  const R w0 = quad_bcast<0>(wd), w1 = quad_bcast<1>(wd), w2 = quad_bcast<2>(wd),
+          w3 = quad_bcast<3>(wd);
+#pragma unroll
+  for (int i = 0; i < 10; i++) {
+    R dn[3], g[3], f[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
+              Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+    sig_dn<R>(dn, s, g);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      f[c] = fma(w0, quad_bcast<0>(g[c]), R(0));
+      f[c] = fma(w1, quad_bcast<1>(g[c]), f[c]);
+      f[c] = fma(w2, quad_bcast<2>(g[c]), f[c]);
+      f[c] = fma(w3, quad_bcast<3>(g[c]), f[c]);
+    }
+    put(i, f);
+  }
+  if constexpr (PHYS != PHYS_COMPAT) {
     R e1[3], e2[3], e3[3];
     double xa[3], va[3];
     get(0, xa, va);

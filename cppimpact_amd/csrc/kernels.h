@@ -9,7 +9,8 @@ namespace cpi {
 
 size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax);
 size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax);
-int elem_block_threads(int npe);
+int elem_block_threads(int npe, int rule);
+int elem_block_default_elements(int npe);
 int elem_direct_threads(int npe);
 
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,

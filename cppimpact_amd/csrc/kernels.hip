@@ -267,6 +267,81 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
+// tet10 on a 4-point rule: four lanes per element, lane t & 3 taking
+// quadrature point t & 3, so a workgroup of NT threads holds at most NT/4
+// elements.  Each point's stress state is loaded before the first barrier.
+// Corner k's element force (summed over the element's four lanes in rule
+// order) is stored by lane k & 3 into the corner's jagged-diagonal slot;
+// the nodes are then assembled as in k_elem_block.  LDS as k_elem_block.
+template <int RULE, typename R, int PHYS, bool AUX, int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_elem_block10(Params p) {
+  extern __shared__ double smem[];
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int t = threadIdx.x, q = t & 3;
+  const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
+  const int32_t *mt = p.meta + 8 * b;
+  const int es = mt[0], nel = mt[1];
+  const int ml = p.ml;
+  const int32_t *nid = p.nid_pad + (long)b * ml;
+  const LNodes ln{smem, ml};
+  R *lf = (R *)((char *)smem + lds_node_bytes(ml));
+  uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 10, p.max_el));
+  const double dt = p.ctl->dt;
+  const bool has_e = (t >> 2) < nel;
+  const int e = es + (t >> 2);
+  PreState<R> pre;
+  if constexpr (PHYS != PHYS_COMPAT) {
+    if (has_e) load_state<R>(p, e, q, pre.s, pre.ep);
+  }
+
+  for (int j = t; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g >= 0) ln.stage(j, p.xm[g], p.vv[g]);
+  }
+  for (int k = t; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
+  __syncthreads();
+
+  EAcc ea;
+  if (has_e) {
+    const uint16_t *lc = p.conn_local + (long)e * 10;
+    const uint16_t *pos = p.csr_pos + (long)e * 10;
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(lc[k], xk, vk); };
+    auto put = [&](int k, const R f[3]) {
+      if ((k & 3) == q) {
+        R *dst = lf + 3 * pos[k];
+        dst[0] = f[0];
+        dst[1] = f[1];
+        dst[2] = f[2];
+      }
+    };
+    tet10_point_lane<RULE, R, PHYS>(p, get, dt, e, q, pre, put, ea);
+  }
+  __syncthreads();
+
+  for (int j = t; j < ml; j += NT) {
+    const int g = nid[j];
+    if (g < 0) continue;
+    double f[3] = {0.0, 0.0, 0.0};
+    const int deg = p.deg_pad[(long)b * ml + j];
+    for (int k = 0; k < deg; k++) {
+      const int qq = joff[k] + j;
+      f[0] += double(lf[3 * qq + 0]);
+      f[1] += double(lf[3 * qq + 1]);
+      f[2] += double(lf[3 * qq + 2]);
+    }
+    const int slot = p.slot_pad[(long)b * ml + j];
+    if (slot < 0) {
+      double x[3], v[3];
+      ln.get(j, x, v);
+      node_update<AUX>(p, g, f, x, v, ln.mass(j), dt, ea);
+    } else {
+      store_partial<R>(p, slot, f);
+    }
+  }
+  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
+}
+
 // tet4: the plan gives every workgroup at most NT elements and NT local
 // nodes, so each thread owns one element and at most one node, and every
 // global load of the workgroup (stress state, local connectivity, the
@@ -1026,7 +1101,16 @@ __global__ void __launch_bounds__(256) k_stream_copy(const Node4 *__restrict__ s
     return hipGetLastError();                                                            \
   } while (0)
 
-template <int NPE> constexpr int block_threads() { return NPE == 4 ? 384 : 128; }
+// threads per workgroup of the block kernels: tet4 one per element (384),
+// tet10 on a 4-point rule one per point (512 for 128 elements), tet10 on
+// the 5-point rule one per element (128)
+template <int NPE, int RULE> constexpr int block_threads() {
+  return NPE == 4 ? 384 : (Quad<RULE>::n == 4 ? 512 : 128);
+}
+// default elements per workgroup
+constexpr int block_elements(int npe) { return npe == 4 ? 384 : 128; }
+// minimum waves per SIMD of the per-point tet10 kernel
+template <typename R> constexpr int tet10_waves() { return 4; }
 
 // minimum waves per SIMD the tet4 element kernels are compiled for: FP64
 // fits 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU).
@@ -1045,13 +1129,18 @@ size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax) {
 }
 
 size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax) {
-  const int nt = npe == 4 ? block_threads<4>() : block_threads<10>();
   size_t s = elem_block_mlp_offset(npe, prec, ml, max_el, kmax);
-  if (phys == PHYS_EP_MLP) s += (size_t)kMlpStageBytes * (nt / 64);
+  // the surrogate runs on tet4 only
+  if (phys == PHYS_EP_MLP) s += (size_t)kMlpStageBytes * (block_threads<4, RULE_CENTROID>() / 64);
   return s;
 }
 
-int elem_block_threads(int npe) { return npe == 4 ? block_threads<4>() : block_threads<10>(); }
+int elem_block_threads(int npe, int rule) {
+  if (npe == 4) return block_threads<4, RULE_CENTROID>();
+  return rule == RULE_REF5 ? block_threads<10, RULE_REF5>() : block_threads<10, RULE_EXACT4>();
+}
+
+int elem_block_default_elements(int npe) { return block_elements(npe); }
 
 // dynamic LDS above 64 KiB needs an explicit opt-in per kernel
 template <typename K>
@@ -1065,7 +1154,7 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
 // stream capture
 template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
-  constexpr int NT = block_threads<NPE>();
+  constexpr int NT = block_threads<NPE, RULE>();
   if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
     constexpr int W = tet4_waves<R, PHYS>();
     if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W>), nblocks, NT, smem, s, p);
@@ -1074,6 +1163,10 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
     constexpr int W = tet4_waves<R, PHYS>();
     if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
     CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
+  } else if constexpr (Quad<RULE>::n == 4) {
+    constexpr int W = tet10_waves<R>();
+    if (aux) CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
   } else {
     if (aux) CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, true, NT>), nblocks, NT, smem, s, p);
     CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, false, NT>), nblocks, NT, smem, s, p);
@@ -1138,7 +1231,7 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
     if constexpr (!valid_combo<NPE, R, PHYS>()) {
       return hipErrorInvalidValue;
     } else {
-      constexpr int NT = block_threads<NPE>();
+      constexpr int NT = block_threads<NPE, RULE>();
       if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
         constexpr int W = tet4_waves<R, PHYS>();
         hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W>, smem);
@@ -1149,6 +1242,11 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
         hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
         if (e != hipSuccess) return e;
         return lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
+      } else if constexpr (Quad<RULE>::n == 4) {
+        constexpr int W = tet10_waves<R>();
+        hipError_t e = lds_opt_in(k_elem_block10<RULE, R, PHYS, true, NT, W>, smem);
+        if (e != hipSuccess) return e;
+        return lds_opt_in(k_elem_block10<RULE, R, PHYS, false, NT, W>, smem);
       } else {
         hipError_t e = lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, true, NT>, smem);
         if (e != hipSuccess) return e;

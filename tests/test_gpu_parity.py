@@ -88,6 +88,35 @@ def test_tet10_matches_oracle(physics, rule, nsteps, scatter):
     g["dyn"].close()
 
 
+@pytest.mark.parametrize("rule", [L.QUAD_EXACT4, L.QUAD_REF4])
+def test_tet10_per_point_block_kernel_matches_colour_scatter(rule):
+    # the 4-point block kernel gives each quadrature point its own lane and
+    # sums the corner forces across the element's lanes; the colour scatter
+    # runs one element per thread.  Same element forces, different node
+    # summation order: FP64 agrees to rounding over a plastic impact on a
+    # mesh of many blocks, and the block runs are bitwise reproducible.
+    x, conn = cube(6, quadratic=True, jitter=0.05, seed=2, gap=1e-6)
+    k = wall_k(0.25, 12)
+    v0, dt, n = (0, 0, -40.0), 1e-7, 60
+    runs = [run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=rule,
+                    scatter=s, use_graph=gr)
+            for s, gr in ((L.SCATTER_BLOCK, 1), (L.SCATTER_BLOCK, 0), (L.SCATTER_COLOR, 1))]
+    blk, blk2, col = runs
+    assert blk["dyn"].info().num_blocks > 4
+    for key in ("x", "v", "a"):
+        assert np.array_equal(blk[key], blk2[key]), key
+    assert _disp_err(blk, col, x) < 1e-11
+    assert rel(blk["v"], col["v"]) < 1e-11
+    sb, eb = blk["dyn"].element_state()
+    sc, ec = col["dyn"].element_state()
+    assert ec.max() > 0.0
+    assert rel(sb, sc) < 1e-10 and rel(eb, ec) < 1e-9
+    for k_ in (O.E_KINETIC, O.E_INTERNAL, O.E_PLASTIC, O.E_PENALTY):
+        assert abs(blk["energy"][k_] - col["energy"][k_]) <= 1e-10 * abs(col["energy"][O.E_KINETIC])
+    for g in runs:
+        g["dyn"].close()
+
+
 def test_f32_elements_within_1e3_of_fp64_oracle():
     x, conn = cube(8, gap=1e-6, jitter=0.1, seed=3)
     k = wall_k(0.25, 8)
