@@ -142,18 +142,26 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
   return xcd * q + min(xcd, r) + k;
 }
 
-// lane K's value of each group of four lanes (4k..4k+3), on all four lanes
-// (DPP quad_perm broadcast; the source lane must be active)
-template <int K>
-__device__ __forceinline__ float quad_bcast(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+// the value of lane (lane ^ 1) (CTRL 0xB1) or (lane ^ 2) (CTRL 0x4E) within
+// each group of four lanes: DPP quad_perm, the source lane must be active
+template <int CTRL>
+__device__ __forceinline__ float quad_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
-template <int K>
-__device__ __forceinline__ double quad_bcast(double v) {
+template <int CTRL>
+__device__ __forceinline__ double quad_swap(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, K * 0x55, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), K * 0x55, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over each group of four lanes, (v0 + v1) + (v2 + v3) on every lane of
+// the group (addition commutes, so all four get the same bits)
+template <typename R>
+__device__ __forceinline__ R quad_sum(R v) {
+  v = v + quad_swap<0xB1>(v);
+  return v + quad_swap<0x4E>(v);
 }
 
 template <typename R>

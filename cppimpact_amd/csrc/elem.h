@@ -281,6 +281,42 @@ __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, doubl
   tet4_forces<R>(Ji, s, wd, put);
 }
 
+// Entries of tet10_grad that are zero at every point (G[3i+c] = dN_i/dxi_c).
+// Products with them are skipped: the compiler may not fold x * 0 away
+// (0 * inf is NaN), so they would cost instructions and registers.
+__host__ __device__ constexpr bool tet10_nz(int k) {
+  return !(k == 1 || k == 2 || k == 3 || k == 5 || k == 9 || k == 10 || k == 14 || k == 22 ||
+           k == 24);
+}
+
+// J[a][c] += G[3i+c] (x_i - o)_a over the non-zero G of node i
+template <typename R>
+__device__ __forceinline__ void tet10_jac_add(R J[9], const double Gd[30], int i, const R d[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (tet10_nz(3 * i + c)) J[a * 3 + c] += R(Gd[3 * i + c]) * d[a];
+}
+
+// dN_i/dx = Ji^T G_i, summed left to right over the non-zero G of node i
+template <typename R>
+__device__ __forceinline__ void tet10_dn(const R Ji[9], const double Gd[30], int i, R dn[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    R acc = R(0);
+    bool first = true;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (!tet10_nz(3 * i + j)) continue;
+      const R t = Ji[j * 3 + c] * R(Gd[3 * i + j]);
+      acc = first ? t : acc + t;
+      first = false;
+    }
+    dn[c] = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // tet10: loop over the rule's points with coordinates relative to corner 2
 // (the origin of the reference's quadratic parametrisation); nodes are
@@ -308,12 +344,8 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
     for (int i = 0; i < 10; i++) {
       double xi[3], vi[3];
       get(i, xi, vi);
-#pragma unroll
-      for (int a = 0; a < 3; a++) {
-        const R d = R(xi[a] - o[a]);
-#pragma unroll
-        for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * d;
-      }
+      const R d[3] = {R(xi[0] - o[0]), R(xi[1] - o[1]), R(xi[2] - o[2])};
+      tet10_jac_add<R>(J, Gd, i, d);
     }
     R Ji[9];
     R detJ = inv3<R>(J, Ji);
@@ -325,10 +357,7 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
       get(i, xi, vi);
       R u[3] = {R(dt * vi[0]), R(dt * vi[1]), R(dt * vi[2])};
       R dn[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
-                Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+      tet10_dn<R>(Ji, Gd, i, dn);
       de[0] += dn[0] * u[0];
       de[1] += dn[1] * u[1];
       de[2] += dn[2] * u[2];
@@ -342,10 +371,7 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 #pragma unroll
     for (int i = 0; i < 10; i++) {
       R dn[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-        dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
-                Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+      tet10_dn<R>(Ji, Gd, i, dn);
       add(i, dn, s, wd);
     }
   }
@@ -371,11 +397,11 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 // tet10 on a 4-point rule with one lane per point: lanes 4k..4k+3 hold the
 // same element e, lane q = (lane & 3) point q.  The point's kinematics and
 // stress are the arithmetic of tet10_points for that point; each corner
-// force is then summed over the quad's lanes in rule order, fused the way
-// add() accumulates it point by point from zero (DPP broadcasts inside the
-// quad), so every lane holds corner i's element force, bit for bit the one
-// of the point loop, and hands it to put(i, f).  All four lanes of a quad
-// must be active.  pre: the point's stress state, loaded ahead (EP models).
+// force w detJ sigma . dN_i is then summed over the quad's lanes as
+// (q0 + q1) + (q2 + q3) (quad_sum, DPP inside the quad), so every lane
+// holds corner i's element force and hands it to put(i, f).  All four
+// lanes of a quad must be active.  pre: the point's stress state, loaded
+// ahead (EP models).
 template <int RULE, typename R, int PHYS, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
@@ -394,12 +420,8 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   for (int i = 0; i < 10; i++) {
     double xi[3], vi[3];
     get(i, xi, vi);
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      const R d = R(xi[a] - o[a]);
-#pragma unroll
-      for (int c = 0; c < 3; c++) J[a * 3 + c] += R(Gd[3 * i + c]) * d;
-    }
+    const R d[3] = {R(xi[0] - o[0]), R(xi[1] - o[1]), R(xi[2] - o[2])};
+    tet10_jac_add<R>(J, Gd, i, d);
   }
   R Ji[9];
   R detJ = inv3<R>(J, Ji);
@@ -411,10 +433,7 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
     get(i, xi, vi);
     R u[3] = {R(dt * vi[0]), R(dt * vi[1]), R(dt * vi[2])};
     R dn[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-      dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
-              Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+    tet10_dn<R>(Ji, Gd, i, dn);
     de[0] += dn[0] * u[0];
     de[1] += dn[1] * u[1];
     de[2] += dn[2] * u[2];
@@ -425,23 +444,13 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
   point_stress<R, PHYS, true>(p, e, q, de, wd, s, ea, &pre);
-  const R w0 = quad_bcast<0>(wd), w1 = quad_bcast<1>(wd), w2 = quad_bcast<2>(wd),
-          w3 = quad_bcast<3>(wd);
 #pragma unroll
   for (int i = 0; i < 10; i++) {
     R dn[3], g[3], f[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-      dn[c] = Ji[0 * 3 + c] * R(Gd[3 * i + 0]) + Ji[1 * 3 + c] * R(Gd[3 * i + 1]) +
-              Ji[2 * 3 + c] * R(Gd[3 * i + 2]);
+    tet10_dn<R>(Ji, Gd, i, dn);
     sig_dn<R>(dn, s, g);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      f[c] = fma(w0, quad_bcast<0>(g[c]), R(0));
-      f[c] = fma(w1, quad_bcast<1>(g[c]), f[c]);
-      f[c] = fma(w2, quad_bcast<2>(g[c]), f[c]);
-      f[c] = fma(w3, quad_bcast<3>(g[c]), f[c]);
-    }
+    for (int c = 0; c < 3; c++) f[c] = quad_sum(wd * g[c]);
     put(i, f);
   }
   if constexpr (PHYS != PHYS_COMPAT) {
