@@ -447,7 +447,20 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
 #pragma unroll
   for (int i = 0; i < 10; i++) {
     R dn[3], g[3], f[3];
-    tet10_dn<R>(Ji, Gd, i, dn);
+    if constexpr (std::is_same<R, double>::value) {
+      // FP64: node i's gradients pass through an empty asm so the strain
+      // pass's dN values are recomputed here, not kept live across the
+      // stress update (60 VGPRs; with them the elastoplastic kernel spills)
+      double Gi[30];
+#pragma unroll
+      for (int k = 0; k < 30; k++) Gi[k] = Gd[k];
+#pragma unroll
+      for (int k = 3 * i; k < 3 * i + 3; k++)
+        if (tet10_nz(k)) __asm__ volatile("" : "+v"(Gi[k]));
+      tet10_dn<R>(Ji, Gi, i, dn);
+    } else {
+      tet10_dn<R>(Ji, Gd, i, dn);
+    }
     sig_dn<R>(dn, s, g);
 #pragma unroll
     for (int c = 0; c < 3; c++) f[c] = quad_sum(wd * g[c]);
