@@ -95,6 +95,21 @@ def train_surrogate(args):
     return net, info
 
 
+def formula_elem_bytes(info, precision):
+    """Algorithmic bytes per launch of the element kernel from the step's
+    compulsory-traffic formula B_step = E (4 npe + 14 nq s) + 224 N:
+    connectivity, sigma (6) and ep (1) read and written per point in the
+    element precision s, and per node the element pass's x, v gather and
+    force scatter (72 B) and the node pass (152 B).  The element kernel's
+    launch covers all of it except the node pass of the block-boundary
+    nodes, which the boundary-node kernel finishes (152 B each)."""
+    from cppimpact_amd import _lib as L
+    s = 4.0 if precision == L.PREC_F32 else 8.0
+    e, n = info.num_elements, info.num_nodes
+    step = e * (4.0 * info.nodes_per_element + 14.0 * info.num_quadrature_pts * s) + 224.0 * n
+    return step - 152.0 * info.num_boundary_nodes
+
+
 def run_mode(x, conn, args, precision, stress_update, dt, net=None):
     import numpy as np
 
@@ -138,8 +153,8 @@ def run_mode(x, conn, args, precision, stress_update, dt, net=None):
         value=ne * args.steps / wall_s, ms_per_step=1e3 * wall_s / args.steps,
         gpu_ms_per_step=gpu_ms / args.steps,
         elem_kernel=elem_name, elem_ms=elem_ms, node_ms=node_ms,
-        elem_bytes=info.elem_kernel_bytes, node_bytes=info.node_kernel_bytes,
-        step_bytes=info.bytes_per_step, x=xg, energy=energy.tolist(), step=step,
+        elem_bytes=formula_elem_bytes(info, precision), model_bytes=info.elem_kernel_bytes,
+        node_bytes=info.node_kernel_bytes, step_bytes=info.bytes_per_step, x=xg, energy=energy.tolist(), step=step,
         epbar_max=float(np.max(ep)), blocks=info.num_blocks,
         boundary_nodes=info.num_boundary_nodes, colors=info.num_colors, ep=ep,
         surrogate_points=stats,
@@ -180,7 +195,7 @@ def run_c3(args, precision, dt):
     n = conn.shape[0]
     return dict(x0=x, x=xg, dt=info.dt, elements=n,
                 value=n * args.c3_steps / wall_s, ms_per_step=1e3 * wall_s / args.c3_steps,
-                elem_ms=elem, elem_gbs=info.elem_kernel_bytes / (elem * 1e-3) / 1e9)
+                elem_ms=elem, elem_gbs=formula_elem_bytes(info, precision) / (elem * 1e-3) / 1e9)
 
 
 def c3_summary(args):
@@ -215,7 +230,9 @@ def roofline(r):
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": r["elem_kernel"], "kernel_ms": round(r["elem_ms"], 5),
-            "algorithmic_bytes": r["elem_bytes"]}
+            "algorithmic_bytes": r["elem_bytes"],
+            "kernel_model_bytes": r["model_bytes"],
+            "kernel_model_gbs": round(r["model_bytes"] / (r["elem_ms"] * 1e-3) / 1e9, 1)}
 
 
 def cpu_baseline(x, conn, dt, nsteps):
