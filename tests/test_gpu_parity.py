@@ -117,6 +117,28 @@ def test_tet10_per_point_block_kernel_matches_colour_scatter(rule):
         g["dyn"].close()
 
 
+def test_tet10_block_size_option():
+    # 4-point tet10 blocks hold at most 128 elements (four lanes each in a
+    # 512-lane workgroup); smaller blocks leave lanes idle and add boundary
+    # nodes but give the same step to rounding
+    x, conn = cube(5, quadratic=True, jitter=0.05, seed=4, gap=1e-6)
+    k = wall_k(0.25, 10)
+    v0, dt, n = (0, 0, -30.0), 1e-7, 40
+    with pytest.raises(L.CpiError):
+        run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=L.QUAD_EXACT4,
+                block_elements=129)
+    ref = run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=L.QUAD_EXACT4)
+    o = run_oracle(x, conn, v0, dt, n, O.EP, k, quad_rule=L.QUAD_EXACT4)
+    for be in (37, 64):
+        g = run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=L.QUAD_EXACT4,
+                    block_elements=be)
+        assert g["dyn"].info().num_blocks > ref["dyn"].info().num_blocks
+        assert _disp_err(g, ref, x) < 1e-11 and rel(g["v"], ref["v"]) < 1e-11
+        assert _disp_err(g, o, x) < 1e-9
+        g["dyn"].close()
+    ref["dyn"].close()
+
+
 def test_f32_elements_within_1e3_of_fp64_oracle():
     x, conn = cube(8, gap=1e-6, jitter=0.1, seed=3)
     k = wall_k(0.25, 8)
