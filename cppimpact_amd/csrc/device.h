@@ -142,17 +142,27 @@ __device__ __forceinline__ int xcd_remap(int hw, int n) {
   return xcd * q + min(xcd, r) + k;
 }
 
-// the value of lane (lane ^ 1) (CTRL 0xB1) or (lane ^ 2) (CTRL 0x4E) within
-// each group of four lanes: DPP quad_perm, the source lane must be active
+// DPP lane moves (the source lane must be active).  CTRL: 0xB1 lane ^ 1 and
+// 0x4E lane ^ 2 (quad_perm), 0x141 the mirror lane within each 8 lanes,
+// 0x140 the mirror lane within each row of 16.
+enum : int { kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140 };
 template <int CTRL>
-__device__ __forceinline__ float quad_swap(float v) {
+__device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 template <int CTRL>
-__device__ __forceinline__ double quad_swap(double v) {
+__device__ __forceinline__ double dpp_mov(double v) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
   const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// lane L's value, the same on every lane
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
@@ -160,8 +170,19 @@ __device__ __forceinline__ double quad_swap(double v) {
 // the group (addition commutes, so all four get the same bits)
 template <typename R>
 __device__ __forceinline__ R quad_sum(R v) {
-  v = v + quad_swap<0xB1>(v);
-  return v + quad_swap<0x4E>(v);
+  v = v + dpp_mov<kDppXor1>(v);
+  return v + dpp_mov<kDppXor2>(v);
+}
+
+// op over each row of 16 lanes with DPP moves (xor 1, xor 2, half-row
+// mirror, row mirror); op commutes, so every lane of a row ends with the
+// same bits
+template <class Op>
+__device__ __forceinline__ double row_reduce(double v, const Op &op) {
+  v = op(v, dpp_mov<kDppXor1>(v));
+  v = op(v, dpp_mov<kDppXor2>(v));
+  v = op(v, dpp_mov<kDppHalfMirror>(v));
+  return op(v, dpp_mov<kDppMirror>(v));
 }
 
 template <typename R>

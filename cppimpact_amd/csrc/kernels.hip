@@ -31,19 +31,21 @@ namespace cpi {
 // reductions
 // ---------------------------------------------------------------------------
 
+// Sum / min over the wave, the same bits on every lane and in every run:
+// each row of 16 lanes is reduced with DPP moves, then the four row values
+// (lanes 0, 16, 32, 48) are combined in a fixed order.  All 64 lanes must
+// be active.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v = row_reduce(v, [](double a, double b) { return a + b; });
+  return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
 }
 __device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-  return v;
+  v = row_reduce(v, [](double a, double b) { return fmin(a, b); });
+  return fmin(fmin(lane_value(v, 0), lane_value(v, 16)), fmin(lane_value(v, 32), lane_value(v, 48)));
 }
 
 // Each wave owns one energy row: its lanes' accumulators are reduced with
-// cross-lane shuffles (fixed order, so deterministic) and lane 0 updates the
+// wave_sum / wave_min (fixed order, so deterministic) and lane 0 updates the
 // row; no workgroup barrier is involved.  Cumulative terms (internal and
 // plastic work, gravity work, inverted-element count) accumulate in the row
 // every step; the instantaneous ones (kinetic and penalty energy, contact
