@@ -44,6 +44,13 @@ __device__ __forceinline__ double wave_min(double v) {
   return fmin(fmin(lane_value(v, 0), lane_value(v, 16)), fmin(lane_value(v, 32), lane_value(v, 48)));
 }
 
+// wave_sum of a term that is zero on every lane of most waves (plastic
+// work, inverted elements, penalty energy, contact force): a uniform branch
+// skips the reduction then
+__device__ __forceinline__ double wave_sum_sparse(double v) {
+  return __ballot(v != 0.0) ? wave_sum(v) : 0.0;
+}
+
 // Each wave owns one energy row: its lanes' accumulators are reduced with
 // wave_sum / wave_min (fixed order, so deterministic) and lane 0 updates the
 // row; no workgroup barrier is involved.  Cumulative terms (internal and
@@ -70,13 +77,13 @@ __device__ __forceinline__ RowPre row_prefetch(const Params &p, int base) {
 template <bool ALL>
 __device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc &ea,
                                            const RowPre &pre) {
-  const double wint = wave_sum(ea.wint), wpl = wave_sum(ea.wpl), bad = wave_sum(ea.bad),
-               wg = wave_sum(ea.wgrav);
+  const double wint = wave_sum(ea.wint), wpl = wave_sum_sparse(ea.wpl),
+               bad = wave_sum_sparse(ea.bad), wg = wave_sum(ea.wgrav);
   double ke = 0.0, pen = 0.0, fc = 0.0, md = 0.0;
   if (ALL) {
     ke = wave_sum(ea.ke);
-    pen = wave_sum(ea.pen);
-    fc = wave_sum(ea.fc);
+    pen = wave_sum_sparse(ea.pen);
+    fc = wave_sum_sparse(ea.fc);
     md = wave_min(ea.mindt);
   }
   if ((threadIdx.x & 63) == 0) {
