@@ -154,6 +154,26 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
   ea.wgrav += m * wg;
 }
 
+// A local node's force from its deg corner-force slots in the block's LDS
+// (jagged-diagonal: the k-th at off[k] + j), summed in slot order in the
+// element precision R: FP64 for FP64 elements, FP32 for FP32 ones, whose
+// element forces carry FP32 rounding already (the nodal update that
+// follows is FP64 either way).
+template <typename R>
+__device__ __forceinline__ void node_force_sum(const R *lf, const uint16_t *joff, int j, int deg,
+                                               double f[3]) {
+  R s[3] = {R(0), R(0), R(0)};
+  for (int k = 0; k < deg; k++) {
+    const int q = joff[k] + j;
+    s[0] += lf[3 * q + 0];
+    s[1] += lf[3 * q + 1];
+    s[2] += lf[3 * q + 2];
+  }
+  f[0] = double(s[0]);
+  f[1] = double(s[1]);
+  f[2] = double(s[2]);
+}
+
 // One block's partial force on a shared node, stored in the element
 // kernel's precision: FP64 records (32 B) for FP64 elements, float4 (16 B)
 // for FP32 ones, whose element forces carry FP32 rounding already.  The
@@ -256,14 +276,9 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
   for (int j = threadIdx.x; j < ml; j += NT) {
     const int g = nid[j];
     if (g < 0) continue;
-    double f[3] = {0.0, 0.0, 0.0};
     const int deg = p.deg_pad[(long)b * ml + j];
-    for (int k = 0; k < deg; k++) {
-      const int q = joff[k] + j;
-      f[0] += double(lf[3 * q + 0]);
-      f[1] += double(lf[3 * q + 1]);
-      f[2] += double(lf[3 * q + 2]);
-    }
+    double f[3];
+    node_force_sum<R>(lf, joff, j, deg, f);
     const int slot = p.slot_pad[(long)b * ml + j];
     if (slot < 0) {
       double x[3], v[3];
@@ -331,14 +346,9 @@ k_elem_block10(Params p) {
   for (int j = t; j < ml; j += NT) {
     const int g = nid[j];
     if (g < 0) continue;
-    double f[3] = {0.0, 0.0, 0.0};
     const int deg = p.deg_pad[(long)b * ml + j];
-    for (int k = 0; k < deg; k++) {
-      const int qq = joff[k] + j;
-      f[0] += double(lf[3 * qq + 0]);
-      f[1] += double(lf[3 * qq + 1]);
-      f[2] += double(lf[3 * qq + 2]);
-    }
+    double f[3];
+    node_force_sum<R>(lf, joff, j, deg, f);
     const int slot = p.slot_pad[(long)b * ml + j];
     if (slot < 0) {
       double x[3], v[3];
@@ -420,13 +430,8 @@ k_elem_block4(Params p) {
   __syncthreads();
 
   if (has_n) {
-    double f[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < deg; k++) {
-      const int q = joff[k] + t;
-      f[0] += double(lf[3 * q + 0]);
-      f[1] += double(lf[3 * q + 1]);
-      f[2] += double(lf[3 * q + 2]);
-    }
+    double f[3];
+    node_force_sum<R>(lf, joff, t, deg, f);
     if (slot < 0) {
       double x[3], v[3];
       ln.get(t, x, v);
@@ -596,13 +601,8 @@ k_elem_block4_mlp(Params p) {
   }
 
   if (has_n) {
-    double f[3] = {0.0, 0.0, 0.0};
-    for (int k = 0; k < deg; k++) {
-      const int q = joff[k] + t;
-      f[0] += double(lf[3 * q + 0]);
-      f[1] += double(lf[3 * q + 1]);
-      f[2] += double(lf[3 * q + 2]);
-    }
+    double f[3];
+    node_force_sum<R>(lf, joff, t, deg, f);
     if (slot < 0) {
       double x[3], v[3];
       ln.get(t, x, v);
