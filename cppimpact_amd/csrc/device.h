@@ -205,13 +205,14 @@ __device__ __forceinline__ R inv3(const R J[9], R Ji[9]) {
 }
 
 // quadratic basis gradients at (x,y,z), tetrahedral.h:199-250 layout
-__device__ __forceinline__ void tet10_grad(double x, double y, double z, double G[30]) {
+template <typename T>
+__device__ __forceinline__ void tet10_grad(T x, T y, T z, T G[30]) {
   G[0] = 4 * x - 1; G[1] = 0.0; G[2] = 0.0;
   G[3] = 0.0; G[4] = 4 * y - 1; G[5] = 0.0;
   G[6] = 4 * x + 4 * y + 4 * z - 3;
   G[7] = 4 * y + 4 * x + 4 * z - 3;
   G[8] = 4 * z + 4 * x + 4 * y - 3;
-  G[9] = 0.0; G[10] = 0.0; G[11] = 4.0 * z - 1.0;
+  G[9] = 0.0; G[10] = 0.0; G[11] = 4 * z - 1;
   G[12] = 4 * y; G[13] = 4 * x; G[14] = 0.0;
   G[15] = -4 * y; G[16] = -8 * y - 4 * x - 4 * z + 4; G[17] = -4 * y;
   G[18] = -8 * x - 4 * y - 4 * z + 4; G[19] = -4 * x; G[20] = -4 * x;

@@ -290,8 +290,8 @@ __host__ __device__ constexpr bool tet10_nz(int k) {
 }
 
 // J[a][c] += G[3i+c] (x_i - o)_a over the non-zero G of node i
-template <typename R>
-__device__ __forceinline__ void tet10_jac_add(R J[9], const double Gd[30], int i, const R d[3]) {
+template <typename R, typename G>
+__device__ __forceinline__ void tet10_jac_add(R J[9], const G Gd[30], int i, const R d[3]) {
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -300,8 +300,8 @@ __device__ __forceinline__ void tet10_jac_add(R J[9], const double Gd[30], int i
 }
 
 // dN_i/dx = Ji^T G_i, summed left to right over the non-zero G of node i
-template <typename R>
-__device__ __forceinline__ void tet10_dn(const R Ji[9], const double Gd[30], int i, R dn[3]) {
+template <typename R, typename G>
+__device__ __forceinline__ void tet10_dn(const R Ji[9], const G Gd[30], int i, R dn[3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     R acc = R(0);
@@ -413,8 +413,9 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
     double ov[3];
     get(2, o, ov);
   }
-  double Gd[30];
-  tet10_grad(Q::pt(q, 0), Q::pt(q, 1), Q::pt(q, 2), Gd);
+  // the point's basis gradients in the element precision
+  R Gd[30];
+  tet10_grad<R>(R(Q::pt(q, 0)), R(Q::pt(q, 1)), R(Q::pt(q, 2)), Gd);
   R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
   for (int i = 0; i < 10; i++) {
@@ -426,12 +427,13 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   R Ji[9];
   R detJ = inv3<R>(J, Ji);
   if (!(detJ > R(0))) ea.bad += 1.0;
+  // strain rate sum_i dN_i (x) v_i, times dt once at the end
   R de[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
   for (int i = 0; i < 10; i++) {
     double xi[3], vi[3];
     get(i, xi, vi);
-    R u[3] = {R(dt * vi[0]), R(dt * vi[1]), R(dt * vi[2])};
+    R u[3] = {R(vi[0]), R(vi[1]), R(vi[2])};
     R dn[3];
     tet10_dn<R>(Ji, Gd, i, dn);
     de[0] += dn[0] * u[0];
@@ -441,6 +443,8 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
     de[4] += dn[2] * u[1] + dn[1] * u[2];
     de[5] += dn[2] * u[0] + dn[0] * u[2];
   }
+#pragma unroll
+  for (int c = 0; c < 6; c++) de[c] *= R(dt);
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
   point_stress<R, PHYS, true>(p, e, q, de, wd, s, ea, &pre);
@@ -451,7 +455,7 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
       // FP64: node i's gradients pass through an empty asm so the strain
       // pass's dN values are recomputed here, not kept live across the
       // stress update (60 VGPRs; with them the elastoplastic kernel spills)
-      double Gi[30];
+      R Gi[30];
 #pragma unroll
       for (int k = 0; k < 30; k++) Gi[k] = Gd[k];
 #pragma unroll
