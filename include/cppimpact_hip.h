@@ -134,7 +134,9 @@ typedef struct cpi_options {
   int32_t use_graph;        /* replay the step as a hipGraph (default 1) */
   int32_t track_energy;     /* per-step energy rows (default 1) */
   int32_t contact_slave_only; /* 0 = every node, as the reference does */
-  int32_t block_elements;   /* elements per workgroup for SCATTER_BLOCK, 0 = auto */
+  int32_t block_elements;   /* elements per workgroup for SCATTER_BLOCK, 0 = auto
+                               (384 tet4, 128 tet10); at most 384 for tet4 and
+                               128 for tet10 on a 4-point rule */
   int32_t device;           /* HIP device ordinal */
   int32_t fixed_bc;         /* 1: hold the mesh's fixed nodes (zero initial
                                velocity, zero acceleration, no contact);
