@@ -2,7 +2,8 @@
 //
 // One loop iteration of Dynamics::solve (dynamics.h:471-511) is, with the
 // LDS-assembled scatter (the default):
-//   k_elem_block4 / k_elem_block
+//   k_elem_block4 (tet4; k_elem_block4_mlp with the surrogate),
+//   k_elem_block10 (tet10, 4-point rules), k_elem_block (tet10, 5 points)
 //                 one workgroup per element block: stage the block's nodes
 //                 in LDS, compute element forces (and the stress update),
 //                 sum them per node in a fixed order, finish the nodes owned
