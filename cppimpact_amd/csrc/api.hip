@@ -45,10 +45,17 @@ struct cpi_ctx {
   int nblocks = 0, n_elem_rows = 0, n_node_rows = 0, n_rows = 0;
   std::vector<int> color_rows;  // first energy row of each colour launch
   size_t smem = 0;
-  hipGraphExec_t g_plain = nullptr, g_aux = nullptr;
-  // kGraphRuns[k] plain steps captured in one graph each
-  hipGraphExec_t g_plain_run[2] = {nullptr, nullptr};
+  // captured steps: one plain step, the energy/output step, and kGraphRuns[k]
+  // plain steps in one graph each; set 1 is the surrogate mode without the
+  // elastic pass (mlp_all), captured only when the context has that pass
+  struct GraphSet {
+    hipGraphExec_t plain = nullptr, aux = nullptr;
+    hipGraphExec_t run[2] = {nullptr, nullptr};
+  } gs[2];
   bool mass_ready = false, inited = false;
+  // surrogate mode: mapped pinned word the elastic pass sets on the first
+  // yield; cpi_step then recaptures its graphs without the elastic pass
+  unsigned *h_plastic = nullptr;
   double dt = 0.0;
   int64_t step = 0;
   MlpHost mlp;  // surrogate weights and sample buffer
@@ -165,7 +172,7 @@ int enqueue_step(cpi_ctx *c, bool aux) {
   return CPI_OK;
 }
 
-// plain steps captured back to back in g_plain_run: one graph launch, and
+// plain steps captured back to back in GraphSet::run: one graph launch, and
 // no gap for the launch between the steps' kernels, per kGraphRuns[k]
 // steps (measured at c2: ≈ 4 µs less per step than one graph per step)
 constexpr int kGraphRuns[2] = {32, 4};
@@ -188,13 +195,28 @@ int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out, int nrep = 1) {
 }
 
 void drop_graphs(cpi_ctx *c) {
-  if (c->g_plain) (void)hipGraphExecDestroy(c->g_plain);
-  if (c->g_aux) (void)hipGraphExecDestroy(c->g_aux);
-  for (auto &g : c->g_plain_run) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
+  for (auto &G : c->gs) {
+    for (hipGraphExec_t *g : {&G.plain, &G.aux, &G.run[0], &G.run[1]}) {
+      if (*g) (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
   }
-  c->g_plain = c->g_aux = nullptr;
+}
+
+// the graph sets of the context's modes (see cpi_ctx::gs)
+int capture_graphs(cpi_ctx *c) {
+  const int saved = c->P.mlp_all;
+  int rc = CPI_OK;
+  for (int m = 0; m < (c->h_plastic ? 2 : 1) && rc == CPI_OK; m++) {
+    auto &G = c->gs[m];
+    c->P.mlp_all = m;
+    if ((rc = capture(c, false, &G.plain)) || (rc = capture(c, true, &G.aux)) ||
+        (rc = capture(c, false, &G.run[0], kGraphRuns[0])) ||
+        (rc = capture(c, false, &G.run[1], kGraphRuns[1])))
+      break;
+  }
+  c->P.mlp_all = saved;
+  return rc;
 }
 
 int ensure_tmp(cpi_ctx *c, size_t n) {
@@ -253,6 +275,8 @@ int reset_state(cpi_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->P.epbar, 0, rs * c->nq * (size_t)c->ne, c->stream));
     if (c->P.yielded)
       HIPCHK(c, hipMemsetAsync(c->P.yielded, 0, 8 * (size_t)c->P.ywords * c->nblocks, c->stream));
+    if (c->P.defer)
+      HIPCHK(c, hipMemsetAsync(c->P.defer, 0, sizeof(unsigned) * c->nblocks, c->stream));
   }
   HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -328,6 +352,7 @@ extern "C" void cpi_destroy(cpi_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   for (void *p : c->allocs) (void)hipFree(p);
+  if (c->h_plastic) (void)hipHostFree(c->h_plastic);
   mlp_free(c->mlp);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -565,6 +590,20 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       if ((rc = dalloc(c, &yw, (size_t)P.ywords * B.nblocks))) return rc;
       P.yielded = yw;
     }
+    // surrogate mode: the elastic pass's deferred-block flags, and the
+    // host word that tells cpi_step a trial stress has yielded
+    if (npe == 4 && c->phys == 2) {
+      unsigned *df = nullptr;
+      if ((rc = dalloc(c, &df, (size_t)B.nblocks))) return rc;
+      P.defer = df;
+      void *hp = nullptr;
+      HIPCHK(c, hipHostMalloc(&hp, sizeof(unsigned), hipHostMallocMapped));
+      c->h_plastic = (unsigned *)hp;
+      *c->h_plastic = 0u;
+      void *dp = nullptr;
+      HIPCHK(c, hipHostGetDevicePointer(&dp, hp, 0));
+      P.plastic_host = (unsigned *)dp;
+    }
     c->smem = elem_block_smem(npe, c->prec, c->phys, P.ml, P.max_el, P.kmax);
     P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.ml, P.max_el, P.kmax);
     if (c->smem > (size_t)prop.sharedMemPerBlock)
@@ -700,6 +739,8 @@ extern "C" int cpi_init(cpi_ctx *c, double dt) {
   }
   c->dt = dt;
   c->step = 0;
+  c->P.mlp_all = 0;
+  if (c->h_plastic) *(volatile unsigned *)c->h_plastic = 0u;
   Ctl h{};
   h.dt = dt;
   HIPCHK(c, hipMemcpyAsync(c->P.ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
@@ -708,12 +749,7 @@ extern "C" int cpi_init(cpi_ctx *c, double dt) {
   if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("init kernels: ") + hipGetErrorString(e));
   HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->opt.use_graph) {
-    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux)) ||
-        (rc = capture(c, false, &c->g_plain_run[0], kGraphRuns[0])) ||
-        (rc = capture(c, false, &c->g_plain_run[1], kGraphRuns[1])))
-      return rc;
-  }
+  if (c->opt.use_graph && (rc = capture_graphs(c))) return rc;
   c->inited = true;
   return CPI_OK;
 }
@@ -724,15 +760,20 @@ extern "C" int cpi_step(cpi_ctx *c, int32_t nsteps) {
   if (nsteps < 0) return fail(c, CPI_ERR_INVALID, "nsteps < 0");
   for (int i = 0; i < nsteps;) {
     const bool aux = (i == nsteps - 1);
-    if (c->g_plain) {
+    // a trial stress has yielded (the device wrote the mapped word): from
+    // here on the surrogate kernel takes every block, the elastic pass would
+    // only defer them.  Either set of graphs is correct at any step.
+    if (c->h_plastic && !c->P.mlp_all && *(volatile unsigned *)c->h_plastic) c->P.mlp_all = 1;
+    const auto &G = c->gs[c->P.mlp_all];
+    if (G.plain) {
       int k = 0;
-      while (k < 2 && !(c->g_plain_run[k] && nsteps - 1 - i >= kGraphRuns[k])) k++;
+      while (k < 2 && !(G.run[k] && nsteps - 1 - i >= kGraphRuns[k])) k++;
       if (!aux && k < 2) {
-        HIPCHK(c, hipGraphLaunch(c->g_plain_run[k], c->stream));
+        HIPCHK(c, hipGraphLaunch(G.run[k], c->stream));
         i += kGraphRuns[k];
         continue;
       }
-      HIPCHK(c, hipGraphLaunch(aux ? c->g_aux : c->g_plain, c->stream));
+      HIPCHK(c, hipGraphLaunch(aux ? G.aux : G.plain, c->stream));
     } else {
       int rc = enqueue_step(c, aux);
       if (rc) return rc;
@@ -1068,13 +1109,7 @@ bool same_value(double a, double b) { return std::fabs(a - b) <= 1e-12 * std::ma
 // re-captures the step graphs after a change of the kernel arguments
 int refresh_graphs(cpi_ctx *c) {
   drop_graphs(c);
-  if (c->inited && c->opt.use_graph) {
-    int rc;
-    if ((rc = capture(c, false, &c->g_plain)) || (rc = capture(c, true, &c->g_aux)) ||
-        (rc = capture(c, false, &c->g_plain_run[0], kGraphRuns[0])) ||
-        (rc = capture(c, false, &c->g_plain_run[1], kGraphRuns[1])))
-      return rc;
-  }
+  if (c->inited && c->opt.use_graph) return capture_graphs(c);
   return CPI_OK;
 }
 

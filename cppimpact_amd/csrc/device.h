@@ -28,7 +28,9 @@ struct Ctl {
   int64_t step;    // completed loop iterations
   int64_t nrows;   // energy rows written to the history ring
   uint32_t flags;  // bit 0: non-finite energy, bit 1: inverted element
-  uint32_t pad;
+  // surrogate mode: 1 + the step in which a trial stress first yielded;
+  // from the step after it on, the elastic pass defers every block
+  uint32_t plastic;
   double e0;       // KE + Epen of the initial state
   double row[8];   // latest energy row (CPI_E_* order)
 };
@@ -106,6 +108,14 @@ struct Params {
   const MlpParams *mlp;  // null unless the surrogate is enabled
   double mlp_tol;        // fallback threshold on |f_yield| / sy0
   unsigned long long *mlp_stats;  // [nblocks][2] surrogate accepted / fell back
+  // surrogate mode, block scatter: 1 for a block the elastic pass left to
+  // the surrogate kernel this step (set by k_elem_block4, cleared by
+  // k_elem_block4_mlp); [nblocks]
+  unsigned *defer;
+  // mapped pinned host word the elastic pass sets when a trial stress
+  // yields (the host then drops the elastic pass: mlp_all)
+  unsigned *plastic_host;
+  int mlp_all;  // the surrogate kernel takes every block, no elastic pass
                                   // per block of the element kernel (optional)
   float *samples;        // return-mapping sample buffer (optional)
   int *sample_count;

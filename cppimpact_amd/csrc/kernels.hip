@@ -374,6 +374,14 @@ k_elem_block4(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
+  if constexpr (PHYS == PHYS_EP_MLP) {
+    // plastic since an earlier step: the surrogate kernel takes every block
+    const uint32_t pl = p.ctl->plastic;
+    if (pl != 0u && (int64_t)pl <= p.ctl->step) {
+      if (t == 0) p.defer[b] = 1u;
+      return;
+    }
+  }
   // the node side needs only b: ids, slots and force ranges at a fixed
   // stride, then the node records
   int g = -1, slot = -1, deg = 0;
@@ -411,6 +419,8 @@ k_elem_block4(Params p) {
   __syncthreads();
 
   EAcc ea;
+  [[maybe_unused]] R snew[6];
+  [[maybe_unused]] bool yielding = false;
   if (has_e) {
     const int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
                       (int)(lc4.y >> 16)};
@@ -422,13 +432,45 @@ k_elem_block4(Params p) {
 #pragma unroll
       for (int c = 0; c < 3; c++) dst[c] = fk[c];
     };
-    tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+    if constexpr (PHYS == PHYS_EP_MLP) {
+      // the elastic pass of the surrogate mode: an element whose trial
+      // stress stays inside the yield surface finishes as in the surrogate
+      // kernel's first phase; a yielding one sends the whole block there
+      R Ji[9], de[6], wd;
+      tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, ea);
+      R wk = ep_trial<R>(p, de, pre.s, snew);
+      if (!(yield_overshoot<R>(p, snew, pre.ep) > R(0))) {
+#pragma unroll
+        for (int c = 0; c < 6; c++) wk += snew[c] * de[c];
+        ea.wint += double(wd * R(0.5) * wk);
+        tet4_forces<R>(Ji, snew, wd, put);
+      } else {
+        yielding = true;
+      }
+    } else {
+      tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+    }
   }
-  if constexpr (PHYS != PHYS_COMPAT) {
+  if constexpr (PHYS == PHYS_EP) {
     const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
     if (p.yielded && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
-  __syncthreads();
+  if constexpr (PHYS == PHYS_EP_MLP) {
+    // nothing of the block has reached global memory yet: a block with a
+    // yielding element goes whole to k_elem_block4_mlp, and from the next
+    // step on every block does (the surrogate mode's plastic phase)
+    if (__syncthreads_or(yielding)) {
+      if (t == 0) {
+        p.defer[b] = 1u;
+        p.ctl->plastic = (uint32_t)(p.ctl->step + 1);
+        if (p.plastic_host) *(volatile unsigned *)p.plastic_host = 1u;
+      }
+      return;
+    }
+    if (has_e) store_state<R>(p, e, 0, snew, pre.ep, false);
+  } else {
+    __syncthreads();
+  }
 
   if (has_n) {
     double f[3];
@@ -468,6 +510,10 @@ k_elem_block4_mlp(Params p) {
   __shared__ unsigned mcnt[2];  // network accepted / fell back
   const int b = xcd_chunked(blockIdx.x, gridDim.x);
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  // every block (mlp_all), or only the blocks the elastic pass
+  // (k_elem_block4 with PHYS_EP_MLP) deferred this step, whose flags are
+  // cleared for the next one
+  if (!p.mlp_all && !p.defer[b]) return;
   int g = -1, slot = -1, deg = 0;
   if (t < p.ml) {
     const long o = (long)b * p.ml + t;
@@ -613,6 +659,7 @@ k_elem_block4_mlp(Params p) {
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
+  if (!p.mlp_all && t == 0) p.defer[b] = 0u;
 }
 
 template <bool AUX, typename R>
@@ -1166,9 +1213,23 @@ template <int NPE, int RULE, typename R, int PHYS>
 static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
   constexpr int NT = block_threads<NPE, RULE>();
   if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
-    constexpr int W = tet4_waves<R, PHYS>();
-    if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W>), nblocks, NT, smem, s, p);
-    CPI_LAUNCH((k_elem_block4_mlp<false, NT, W>), nblocks, NT, smem, s, p);
+    // the elastic pass at the plain FP32 kernel's occupancy (no network
+    // staging in its LDS), then the surrogate kernel over the blocks it
+    // deferred (all of them once any trial stress has yielded)
+    constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
+    const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
+    if (p.mlp_all) {
+      if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W>), nblocks, NT, smem, s, p);
+      CPI_LAUNCH((k_elem_block4_mlp<false, NT, W>), nblocks, NT, smem, s, p);
+    }
+    if (aux) {
+      hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+      hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+    } else {
+      hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+      hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+    }
+    return hipGetLastError();
   } else if constexpr (NPE == 4) {
     constexpr int W = tet4_waves<R, PHYS>();
     if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
@@ -1243,10 +1304,13 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
     } else {
       constexpr int NT = block_threads<NPE, RULE>();
       if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
-        constexpr int W = tet4_waves<R, PHYS>();
+        constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
+        const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
         hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, WA>, smem_a);
         if (e != hipSuccess) return e;
-        return lds_opt_in(k_elem_block4_mlp<false, NT, W>, smem);
+        return lds_opt_in(k_elem_block4<R, PHYS, false, NT, WA>, smem_a);
       } else if constexpr (NPE == 4) {
         constexpr int W = tet4_waves<R, PHYS>();
         hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
