@@ -194,6 +194,14 @@ int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out, int nrep = 1) {
   return CPI_OK;
 }
 
+// Surrogate mode: once a trial stress has yielded (the elastic pass wrote
+// the mapped host word), the surrogate kernel takes every block; the
+// elastic pass would defer most of them to it.  Either launch form is
+// correct at any step, so seeing the word late only costs time.
+void mlp_mode_update(cpi_ctx *c) {
+  if (c->h_plastic && !c->P.mlp_all && *(volatile unsigned *)c->h_plastic) c->P.mlp_all = 1;
+}
+
 void drop_graphs(cpi_ctx *c) {
   for (auto &G : c->gs) {
     for (hipGraphExec_t *g : {&G.plain, &G.aux, &G.run[0], &G.run[1]}) {
@@ -604,6 +612,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       HIPCHK(c, hipHostGetDevicePointer(&dp, hp, 0));
       P.plastic_host = (unsigned *)dp;
     }
+    P.nblocks = B.nblocks;
     c->smem = elem_block_smem(npe, c->prec, c->phys, P.ml, P.max_el, P.kmax);
     P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.ml, P.max_el, P.kmax);
     if (c->smem > (size_t)prop.sharedMemPerBlock)
@@ -760,10 +769,7 @@ extern "C" int cpi_step(cpi_ctx *c, int32_t nsteps) {
   if (nsteps < 0) return fail(c, CPI_ERR_INVALID, "nsteps < 0");
   for (int i = 0; i < nsteps;) {
     const bool aux = (i == nsteps - 1);
-    // a trial stress has yielded (the device wrote the mapped word): from
-    // here on the surrogate kernel takes every block, the elastic pass would
-    // only defer them.  Either set of graphs is correct at any step.
-    if (c->h_plastic && !c->P.mlp_all && *(volatile unsigned *)c->h_plastic) c->P.mlp_all = 1;
+    mlp_mode_update(c);
     const auto &G = c->gs[c->P.mlp_all];
     if (G.plain) {
       int k = 0;
@@ -984,6 +990,7 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
   hipEvent_t ev[3];
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
   for (int i = 0; i < nsteps; i++) {
+    mlp_mode_update(c);
     Params p = c->P;
     p.store_aux = 0;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));

@@ -28,9 +28,7 @@ struct Ctl {
   int64_t step;    // completed loop iterations
   int64_t nrows;   // energy rows written to the history ring
   uint32_t flags;  // bit 0: non-finite energy, bit 1: inverted element
-  // surrogate mode: 1 + the step in which a trial stress first yielded;
-  // from the step after it on, the elastic pass defers every block
-  uint32_t plastic;
+  uint32_t pad;
   double e0;       // KE + Epen of the initial state
   double row[8];   // latest energy row (CPI_E_* order)
 };
@@ -116,6 +114,7 @@ struct Params {
   // yields (the host then drops the elastic pass: mlp_all)
   unsigned *plastic_host;
   int mlp_all;  // the surrogate kernel takes every block, no elastic pass
+  int nblocks;  // element blocks of the block scatter
                                   // per block of the element kernel (optional)
   float *samples;        // return-mapping sample buffer (optional)
   int *sample_count;

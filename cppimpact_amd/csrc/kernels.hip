@@ -374,14 +374,6 @@ k_elem_block4(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int t = threadIdx.x;
-  if constexpr (PHYS == PHYS_EP_MLP) {
-    // plastic since an earlier step: the surrogate kernel takes every block
-    const uint32_t pl = p.ctl->plastic;
-    if (pl != 0u && (int64_t)pl <= p.ctl->step) {
-      if (t == 0) p.defer[b] = 1u;
-      return;
-    }
-  }
   // the node side needs only b: ids, slots and force ranges at a fixed
   // stride, then the node records
   int g = -1, slot = -1, deg = 0;
@@ -457,12 +449,11 @@ k_elem_block4(Params p) {
   }
   if constexpr (PHYS == PHYS_EP_MLP) {
     // nothing of the block has reached global memory yet: a block with a
-    // yielding element goes whole to k_elem_block4_mlp, and from the next
-    // step on every block does (the surrogate mode's plastic phase)
+    // yielding element goes whole to k_elem_block4_mlp, and the host is
+    // told (it then launches the surrogate kernel alone over every block)
     if (__syncthreads_or(yielding)) {
       if (t == 0) {
         p.defer[b] = 1u;
-        p.ctl->plastic = (uint32_t)(p.ctl->step + 1);
         if (p.plastic_host) *(volatile unsigned *)p.plastic_host = 1u;
       }
       return;
@@ -500,20 +491,12 @@ k_elem_block4(Params p) {
 //            return on failure), then the element's work, state and forces.
 // LDS: node records, corner forces, one MLP staging area per wave, the
 // queue and the per-wave counts.
-template <bool AUX, int NT, int WPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_elem_block4_mlp(Params p) {
+template <bool AUX, int NT>
+__device__ __forceinline__ void mlp_block(const Params &p, const int b, uint16_t *queue, int *wcnt,
+                                          unsigned *mcnt) {
   using R = float;
   extern __shared__ double smem[];
-  __shared__ uint16_t queue[NT];
-  __shared__ int wcnt[NT / 64];
-  __shared__ unsigned mcnt[2];  // network accepted / fell back
-  const int b = xcd_chunked(blockIdx.x, gridDim.x);
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  // every block (mlp_all), or only the blocks the elastic pass
-  // (k_elem_block4 with PHYS_EP_MLP) deferred this step, whose flags are
-  // cleared for the next one
-  if (!p.mlp_all && !p.defer[b]) return;
   int g = -1, slot = -1, deg = 0;
   if (t < p.ml) {
     const long o = (long)b * p.ml + t;
@@ -659,7 +642,44 @@ k_elem_block4_mlp(Params p) {
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
-  if (!p.mlp_all && t == 0) p.defer[b] = 0u;
+}
+
+// LIST = false: one block per workgroup, every block (the surrogate kernel
+// alone does the step's element pass).  LIST = true: the blocks the elastic
+// pass (k_elem_block4 with PHYS_EP_MLP) deferred this step; workgroup i of
+// a small grid reads the flags of blocks i, i + grid, ... (K <= 64 of them)
+// in one wave, takes the flagged ones in order and clears their flags
+// (until the first yield none is, and the launch only reads flags; the
+// stride spreads a cluster of yielding blocks over the workgroups).
+template <bool AUX, int NT, int WPE, bool LIST>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_elem_block4_mlp(Params p) {
+  __shared__ uint16_t queue[NT];
+  __shared__ int wcnt[NT / 64];
+  __shared__ unsigned mcnt[2];  // network accepted / fell back
+  if constexpr (!LIST) {
+    mlp_block<AUX, NT>(p, xcd_chunked(blockIdx.x, gridDim.x), queue, wcnt, mcnt);
+  } else {
+    __shared__ unsigned long long lmask;
+    const int K = (p.nblocks + (int)gridDim.x - 1) / (int)gridDim.x;
+    if (threadIdx.x < 64) {
+      const int b = (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
+      const bool f = (int)threadIdx.x < K && b < p.nblocks && p.defer[b] != 0u;
+      const unsigned long long m = __ballot(f);
+      if (threadIdx.x == 0) lmask = m;
+    }
+    __syncthreads();
+    unsigned long long m = lmask;
+    while (m) {
+      const int b = (int)blockIdx.x + (__ffsll((long long)m) - 1) * (int)gridDim.x;
+      m &= m - 1ull;
+      mlp_block<AUX, NT>(p, b, queue, wcnt, mcnt);
+      // every thread is past the block's last LDS read before the next
+      // block's staging
+      __syncthreads();
+      if (threadIdx.x == 0) p.defer[b] = 0u;
+    }
+  }
 }
 
 template <bool AUX, typename R>
@@ -1174,6 +1194,10 @@ template <typename R> constexpr int tet10_waves() { return 4; }
 // The surrogate kernel's LDS (MLP staging) allows four workgroups per CU,
 // i.e. six waves per SIMD, so it may use 80 VGPRs: with them its spill code
 // stays inside the network phase, off the elastic path.
+// workgroups of the surrogate kernel that walk the blocks the elastic pass
+// deferred: one per CU
+constexpr int kMlpListGrid = 256;
+
 template <typename R, int PHYS> constexpr int tet4_waves() {
   return PHYS == PHYS_EP_MLP ? 6 : std::is_same<R, double>::value ? 5 : 8;
 }
@@ -1219,15 +1243,17 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
     constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
     const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
     if (p.mlp_all) {
-      if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W>), nblocks, NT, smem, s, p);
-      CPI_LAUNCH((k_elem_block4_mlp<false, NT, W>), nblocks, NT, smem, s, p);
+      if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W, false>), nblocks, NT, smem, s, p);
+      CPI_LAUNCH((k_elem_block4_mlp<false, NT, W, false>), nblocks, NT, smem, s, p);
     }
+    // each workgroup of the list walk covers at most 64 blocks
+    const int grid_list = std::max(std::min(nblocks, kMlpListGrid), (nblocks + 63) / 64);
     if (aux) {
       hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
-      hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+      hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
     } else {
       hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
-      hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+      hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
     }
     return hipGetLastError();
   } else if constexpr (NPE == 4) {
@@ -1306,8 +1332,10 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
       if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
         constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
         const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
-        hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W>, smem);
+        hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W, false>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W, false>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<true, NT, W, true>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W, true>, smem);
         if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, WA>, smem_a);
         if (e != hipSuccess) return e;
         return lds_opt_in(k_elem_block4<R, PHYS, false, NT, WA>, smem_a);
