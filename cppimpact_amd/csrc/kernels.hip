@@ -660,16 +660,12 @@ k_elem_block4_mlp(Params p) {
   if constexpr (!LIST) {
     mlp_block<AUX, NT>(p, xcd_chunked(blockIdx.x, gridDim.x), queue, wcnt, mcnt);
   } else {
-    __shared__ unsigned long long lmask;
+    // every wave reads the same flags, so all agree on the list without a
+    // barrier before the first block
     const int K = (p.nblocks + (int)gridDim.x - 1) / (int)gridDim.x;
-    if (threadIdx.x < 64) {
-      const int b = (int)blockIdx.x + (int)threadIdx.x * (int)gridDim.x;
-      const bool f = (int)threadIdx.x < K && b < p.nblocks && p.defer[b] != 0u;
-      const unsigned long long m = __ballot(f);
-      if (threadIdx.x == 0) lmask = m;
-    }
-    __syncthreads();
-    unsigned long long m = lmask;
+    const int lane = (int)(threadIdx.x & 63);
+    const int bl = (int)blockIdx.x + lane * (int)gridDim.x;
+    unsigned long long m = __ballot(lane < K && bl < p.nblocks && p.defer[bl] != 0u);
     while (m) {
       const int b = (int)blockIdx.x + (__ffsll((long long)m) - 1) * (int)gridDim.x;
       m &= m - 1ull;
