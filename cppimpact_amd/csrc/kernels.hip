@@ -367,7 +367,10 @@ k_elem_block10(Params p) {
 // global load of the workgroup (stress state, local connectivity, the
 // contribution lists, node records, slots) is issued before the first
 // barrier.  LDS: node records (LNodes), corner forces [max_el*4][3] R in
-// jagged-diagonal slots, the slot offsets off[k].
+// jagged-diagonal slots, the slot offsets off[k].  PHYS_EP_MLP is the
+// surrogate mode's elastic pass: a block whose trial stresses all stay
+// elastic finishes here, one with a yielding element is deferred whole to
+// k_elem_block4_mlp before it writes anything.
 template <typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
