@@ -227,3 +227,37 @@ def test_surrogate_step_is_bitwise_reproducible():
             assert np.array_equal(a, b)
         assert runs[0][4] == other[4]
     net.close()
+
+
+def test_surrogate_elastic_pass_handover_is_bitwise_stable():
+    """Surrogate steps run an elastic pass until the first yield, hand
+    yielding blocks to the surrogate kernel, and switch to the surrogate
+    kernel alone once the host sees the first yield.  When the host sees it
+    must not matter: one 80-step call (the switch comes late, deferred
+    blocks go through the list walk meanwhile), 80 one-step calls (the
+    switch comes at once) and an eager run agree bit for bit on a mesh of
+    many blocks, counters included."""
+    x, conn = cube(10, gap=2e-6, jitter=0.1, seed=5)
+    smp = collect_samples(ci.Mesh.from_arrays(x, conn), material(),
+                          ci.Wall("Wall", 0.0, wall_k(0.25, 6), None, 0, 1), [30.0], 0.0, 40)
+    net = Surrogate(material(), seed=4)
+    net.train(smp, epochs=3, batch=512)
+    runs = []
+    for graph, chunk in ((1, 80), (1, 1), (0, 80)):
+        d = _impact(x, conn, 30.0, stress_update=L.RETURN_MLP, use_graph=graph)
+        d.set_surrogate(net)
+        d.init(0.0)
+        assert d.info().num_blocks >= 12
+        for _ in range(80 // chunk):
+            d.step(chunk)
+        xs, vs, acc, *_ = d.fetch()
+        _, ep = d.element_state()
+        runs.append((xs, vs, acc, ep, d.surrogate_stats(), d.energy()[0]))
+        d.close()
+    assert runs[0][4][0] > 0 and runs[0][3].max() > 0.0
+    for other in runs[1:]:
+        for a, b in zip(runs[0][:4], other[:4]):
+            assert np.array_equal(a, b)
+        assert runs[0][4] == other[4]
+        assert np.allclose(runs[0][5], other[5], rtol=1e-12, atol=0.0)
+    net.close()
