@@ -121,20 +121,37 @@ class Surrogate:
 def collect_samples(mesh, material, wall, speeds, dt, nsteps, capacity=1 << 21,
                     direction=(0.0, 0.0, -1.0), **options):
     """Runs exact-return impacts (FP32 element kernels unless overridden) at
-    each speed and returns the recorded yielding points [k][20]."""
+    each speed and returns the recorded yielding points [k][20], sorted.
+
+    capacity is the first recorder size per speed.  The recorder fills its
+    slots in atomic-counter order, so a run that sees more yielding points
+    than it holds would keep a subset that depends on that order: such a
+    speed is run again with room for every point it saw (the simulation is
+    deterministic, so the count is), and the set, and the training run on
+    it, are the same in every process."""
     out = []
     opts = dict(physics=L.PHYSICS_EP, precision=L.PREC_F32)
     opts.update(options)
-    for v in speeds:
+
+    def run(v, cap):
         dyn = Dynamics(mesh, material, wall, **opts)
         dyn.vel[:] = np.asarray(direction, dtype=np.float64) * float(v)
-        dyn.enable_samples(capacity)
+        dyn.enable_samples(cap)
         dyn.init(dt)
         dyn.step(nsteps)
         dyn.synchronize()
-        s, _ = dyn.samples()
-        out.append(s)
+        got = dyn.samples()
         dyn.close()
+        return got
+
+    for v in speeds:
+        s, seen = run(v, capacity)
+        if seen > len(s):
+            s, seen2 = run(v, seen)
+            if seen2 != seen or len(s) != seen:
+                raise RuntimeError(f"collect_samples: {seen} then {seen2} yielding points at "
+                                   f"{v} m/s, {len(s)} kept")
+        out.append(s)
     if not out:
         return np.zeros((0, L.SAMPLE_FLOATS), np.float32)
     smp = np.concatenate(out)
