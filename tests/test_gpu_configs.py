@@ -5,6 +5,10 @@ c0: 10^3-hex cube, 6,000 tet4, 1,000 steps at -10 m/s, FP64, against the
 c1: 20^3-hex cube, 48,000 tet10 (4-point rule), -20 m/s: the first steps
     against the oracle on the full mesh, then 2,000 GPU steps checked
     through the energy balance and FP32 against FP64.
+c2: 64^3-hex cube, 1,572,864 tet4 jittered by 0.1 of the spacing (seed 0),
+    -10 m/s, the throughput workload: the first steps against the oracle on
+    the full mesh, then 241 steps of FP64, FP32 and FP32 with the surrogate
+    (the impact stays elastic, so the network is never evaluated).
 c3: thin-walled cylinder R 1 m, L 4 m, t 0.02 m, 360 x 185 x 3 cells x 6
     tet10 (1,198,800 elements), oblique 7 m/s impact 30 degrees from the
     wall normal, 1,000 steps, FP32 against FP64.
@@ -123,6 +127,67 @@ def test_c1_two_thousand_steps_energy_and_fp32():
     d64.close()
     d32.close()
     dcol.close()
+
+
+def c2_mesh():
+    x, conn = ci.box_mesh(64, 64, 64, 0.25, 0.25, 0.25, origin=(0.0, 0.0, 1e-6), jitter=0.1,
+                          seed=0)
+    assert conn.shape == (1572864, 4) and x.shape[0] == 274625
+    return x, conn, 10 * E * 0.25 / 64
+
+
+def test_c2_first_steps_against_oracle_on_full_mesh():
+    x, conn, k = c2_mesh()
+    v0 = (0.0, 0.0, -10.0)
+    d = dynamics(x, conn, v0, k)
+    g = run(d, 0.0, 3)
+    o = O.Solver(x, conn, v0, E=E, nu=NU, rho=RHO, sigma_y=SY, H=H, physics=O.EP,
+                 wall=dict(location=0.0, stiffness=k), dt=g["dt"]).solve(3)
+    assert rel(g["m"], o["m"]) < 1e-12
+    assert rel(g["x"] - x, o["x"] - x) < 1e-9
+    assert rel(g["v"], o["v"]) < 1e-9
+    d.close()
+
+
+def test_c2_fp32_and_surrogate_modes_against_fp64():
+    from cppimpact_amd.surrogate import Surrogate
+
+    x, conn, k = c2_mesh()
+    v0 = (0.0, 0.0, -10.0)
+    d64 = dynamics(x, conn, v0, k)
+    d32 = dynamics(x, conn, v0, k, precision=L.PREC_F32)
+    dml = dynamics(x, conn, v0, k, precision=L.PREC_F32, stress_update=L.RETURN_MLP)
+    # an untrained network with unit scaling: a point that reached it would
+    # show in the stats
+    net = Surrogate(material(), seed=0)
+    net.set_scaling(np.zeros(L.MLP_INPUTS), np.ones(L.MLP_INPUTS), np.zeros(L.MLP_OUTPUTS),
+                    np.ones(L.MLP_OUTPUTS))
+    dml.set_surrogate(net)
+    d64.init(0.0)
+    for d in (d32, dml):
+        d.init(d64.info().dt)
+    for d in (d64, d32, dml):
+        d.step(241)
+    x64, x32, xml = d64.fetch()[0], d32.fetch()[0], dml.fetch()[0]
+    _, ep64 = d64.element_state()
+    _, epml = dml.element_state()
+    m = d64.fetch()[3]
+    ke0 = 0.5 * m.sum() * 10.0 ** 2
+    e, _ = d64.energy()
+    err32, errml = rel(x32 - x, x64 - x), rel(xml - x, x64 - x)
+    print(f"c2: 241 steps, fp32 vs fp64 disp {err32:.2e}, surrogate mode vs fp64 {errml:.2e}, "
+          f"surrogate vs fp32 {rel(xml - x, x32 - x):.2e}, balance {e[L.E_BALANCE] / ke0:.2e} KE0, "
+          f"stats {dml.surrogate_stats()}")
+    assert ep64.max() == 0.0 and epml.max() == 0.0  # elastic at 10 m/s
+    assert dml.surrogate_stats() == (0, 0)
+    # FP32 element arithmetic with FP64 nodal state; measured 2.5e-5
+    assert err32 < 1e-3 and errml < 1e-3
+    # the surrogate mode's elastic pass rounds as the FP32 kernel's elastic
+    # path does: the same positions bit for bit
+    assert np.array_equal(xml, x32)
+    assert np.isfinite(e).all() and abs(e[L.E_BALANCE]) < 1e-2 * ke0
+    for d in (d64, d32, dml):
+        d.close()
 
 
 def test_c3_cylinder_oblique_impact_fp32_against_fp64():
