@@ -190,6 +190,27 @@ def test_c2_fp32_and_surrogate_modes_against_fp64():
         d.close()
 
 
+@pytest.mark.parametrize("precision", [L.PREC_F64, L.PREC_F32], ids=["f64", "f32"])
+def test_c2_graph_replay_equals_eager_launches(precision):
+    """The throughput workload as bench.py times it (steps replayed from
+    captured multi-step graphs) against the same steps launched one kernel at
+    a time: the same positions and velocities bit for bit."""
+    x, conn, k = c2_mesh()
+    v0 = (0.0, 0.0, -10.0)
+    dg = dynamics(x, conn, v0, k, precision=precision)
+    de = dynamics(x, conn, v0, k, precision=precision, use_graph=0)
+    dg.init(0.0)
+    de.init(dg.info().dt)
+    for d in (dg, de):
+        d.step(70)
+    xg, vg = dg.fetch()[:2]
+    xe, ve = de.fetch()[:2]
+    assert np.array_equal(xg, xe) and np.array_equal(vg, ve)
+    assert np.abs(xg - x).max() > 0  # the cube has moved
+    dg.close()
+    de.close()
+
+
 def test_c3_cylinder_oblique_impact_fp32_against_fp64():
     nt, nz, nr = 360, 185, 3
     R, Lz, t = 1.0, 4.0, 0.02
