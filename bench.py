@@ -83,7 +83,7 @@ def train_surrogate(args):
     speeds = np.random.default_rng(0).uniform(5.0, 30.0, 6)
     t0 = time.time()
     smp = collect_samples(ci.Mesh.from_arrays(x, conn), mat, wall, speeds, 0.0, 400,
-                          capacity=1 << 20)
+                          capacity=1 << 21)
     t1 = time.time()
     net = Surrogate(mat, seed=0)
     hist = net.train(smp, epochs=args.train_epochs, batch=1024)
