@@ -13,8 +13,12 @@ int elem_block_threads(int npe, int rule);
 int elem_block_default_elements(int npe);
 int elem_direct_threads(int npe);
 
+// part: 0 = the whole element pass; with the surrogate's elastic pass
+// (PHYS_EP_MLP before the switch to the surrogate kernel alone) 1 launches
+// only the elastic pass and 2 only the deferred-block walk, so each can be
+// timed on its own
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,
-                             int nblocks, size_t smem, hipStream_t s);
+                             int nblocks, size_t smem, hipStream_t s, int part = 0);
 hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem);
 hipError_t launch_elem_direct(const Params &p, int rule, int prec, int phys, bool atomic,
                               hipStream_t s);

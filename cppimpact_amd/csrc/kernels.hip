@@ -1233,7 +1233,8 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
 // the LDS opt-in (prepare_elem_block) is done once at set-up, outside any
 // stream capture
 template <int NPE, int RULE, typename R, int PHYS>
-static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s) {
+static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s,
+                               int part) {
   constexpr int NT = block_threads<NPE, RULE>();
   if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
     // the elastic pass at the plain FP32 kernel's occupancy (no network
@@ -1247,12 +1248,17 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
     }
     // each workgroup of the list walk covers at most 64 blocks
     const int grid_list = std::max(std::min(nblocks, kMlpListGrid), (nblocks + 63) / 64);
-    if (aux) {
-      hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
-      hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
-    } else {
-      hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
-      hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
+    if (part != 2) {
+      if (aux)
+        hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+      else
+        hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+    }
+    if (part != 1) {
+      if (aux)
+        hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
+      else
+        hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
     }
     return hipGetLastError();
   } else if constexpr (NPE == 4) {
@@ -1315,11 +1321,11 @@ constexpr bool valid_combo() {
 }
 
 template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockF {
-  static hipError_t run(const Params &p, bool aux, int nb, size_t smem, hipStream_t s) {
+  static hipError_t run(const Params &p, bool aux, int nb, size_t smem, hipStream_t s, int part) {
     if constexpr (!valid_combo<NPE, R, PHYS>())
       return hipErrorInvalidValue;
     else
-      return elem_block_t<NPE, RULE, R, PHYS>(p, aux, nb, smem, s);
+      return elem_block_t<NPE, RULE, R, PHYS>(p, aux, nb, smem, s, part);
   }
 };
 template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
@@ -1366,8 +1372,8 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemDirectF {
 };
 
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux, int nblocks,
-                             size_t smem, hipStream_t s) {
-  return dispatch<ElemBlockF>(p.npe, rule, prec, phys, p, aux, nblocks, smem, s);
+                             size_t smem, hipStream_t s, int part) {
+  return dispatch<ElemBlockF>(p.npe, rule, prec, phys, p, aux, nblocks, smem, s, part);
 }
 
 hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem) {

@@ -244,7 +244,11 @@ int cpi_get_coloring(cpi_ctx *ctx, int32_t *color, int32_t *num_colors);
 int cpi_time_steps(cpi_ctx *ctx, int32_t nsteps, double *elapsed_ms);
 
 /* Runs nsteps steps launching each kernel separately with HIP events
-   around every launch on the context's stream; fills per-kernel totals. */
+   around every launch on the context's stream; fills per-kernel totals:
+   the element pass ("elem_block" / "elem_direct") and the node pass
+   ("node_bnd" / "node_all"), then, in the surrogate mode before the switch
+   to the surrogate kernel alone, its two launches one by one ("elem_pass",
+   "mlp_walk").  *count = entries written (at most cap). */
 int cpi_profile_kernels(cpi_ctx *ctx, int32_t nsteps, cpi_kernel_time *out,
                         int32_t cap, int32_t *count);
 
