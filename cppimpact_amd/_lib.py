@@ -57,7 +57,8 @@ class Options(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "physics", "precision", "scatter", "quadrature", "mass", "stress_update",
         "use_graph", "track_energy", "contact_slave_only", "block_elements", "device",
-        "fixed_bc")] + [("gravity", C.c_double * 3), ("mlp_tolerance", C.c_double)]
+        "fixed_bc", "debug_fill_nan")] + [("gravity", C.c_double * 3),
+                                          ("mlp_tolerance", C.c_double)]
 
 
 class Info(C.Structure):
@@ -152,6 +153,7 @@ SIGNATURES = {
                                           C.POINTER(C.c_int64)]),
     "cpi_reset_surrogate_stats": (C.c_int, [C.c_void_p]),
     "cpi_samples_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "cpi_samples_subsample": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64]),
     "cpi_samples_get": (C.c_int, [C.c_void_p, _fp, C.c_int32, _ip]),
     "cpi_samples_reset": (C.c_int, [C.c_void_p]),
 }

@@ -46,16 +46,12 @@ struct cpi_ctx {
   std::vector<int> color_rows;  // first energy row of each colour launch
   size_t smem = 0;
   // captured steps: one plain step, the energy/output step, and kGraphRuns[k]
-  // plain steps in one graph each; set 1 is the surrogate mode without the
-  // elastic pass (mlp_all), captured only when the context has that pass
+  // plain steps in one graph each
   struct GraphSet {
     hipGraphExec_t plain = nullptr, aux = nullptr;
     hipGraphExec_t run[2] = {nullptr, nullptr};
-  } gs[2];
+  } gs;
   bool mass_ready = false, inited = false;
-  // surrogate mode: mapped pinned word the elastic pass sets on the first
-  // yield; cpi_step then recaptures its graphs without the elastic pass
-  unsigned *h_plastic = nullptr;
   double dt = 0.0;
   int64_t step = 0;
   MlpHost mlp;  // surrogate weights and sample buffer
@@ -89,6 +85,8 @@ int dalloc(cpi_ctx *c, T **p, size_t n) {
   }
   c->allocs.push_back(q);
   *p = (T *)q;
+  if (c->opt.debug_fill_nan && c->stream)
+    HIPCHK(c, hipMemsetAsync(q, 0xFF, std::max<size_t>(n, 1) * sizeof(T), c->stream));
   return CPI_OK;
 }
 
@@ -194,37 +192,23 @@ int capture(cpi_ctx *c, bool aux, hipGraphExec_t *out, int nrep = 1) {
   return CPI_OK;
 }
 
-// Surrogate mode: once a trial stress has yielded (the elastic pass wrote
-// the mapped host word), the surrogate kernel takes every block; the
-// elastic pass would defer most of them to it.  Either launch form is
-// correct at any step, so seeing the word late only costs time.
-void mlp_mode_update(cpi_ctx *c) {
-  if (c->h_plastic && !c->P.mlp_all && *(volatile unsigned *)c->h_plastic) c->P.mlp_all = 1;
-}
-
 void drop_graphs(cpi_ctx *c) {
-  for (auto &G : c->gs) {
-    for (hipGraphExec_t *g : {&G.plain, &G.aux, &G.run[0], &G.run[1]}) {
-      if (*g) (void)hipGraphExecDestroy(*g);
-      *g = nullptr;
-    }
+  auto &G = c->gs;
+  for (hipGraphExec_t *g : {&G.plain, &G.aux, &G.run[0], &G.run[1]}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
   }
 }
 
-// the graph sets of the context's modes (see cpi_ctx::gs)
+// the step graphs (see cpi_ctx::gs)
 int capture_graphs(cpi_ctx *c) {
-  const int saved = c->P.mlp_all;
-  int rc = CPI_OK;
-  for (int m = 0; m < (c->h_plastic ? 2 : 1) && rc == CPI_OK; m++) {
-    auto &G = c->gs[m];
-    c->P.mlp_all = m;
-    if ((rc = capture(c, false, &G.plain)) || (rc = capture(c, true, &G.aux)) ||
-        (rc = capture(c, false, &G.run[0], kGraphRuns[0])) ||
-        (rc = capture(c, false, &G.run[1], kGraphRuns[1])))
-      break;
-  }
-  c->P.mlp_all = saved;
-  return rc;
+  auto &G = c->gs;
+  int rc;
+  if ((rc = capture(c, false, &G.plain)) || (rc = capture(c, true, &G.aux)) ||
+      (rc = capture(c, false, &G.run[0], kGraphRuns[0])) ||
+      (rc = capture(c, false, &G.run[1], kGraphRuns[1])))
+    return rc;
+  return CPI_OK;
 }
 
 int ensure_tmp(cpi_ctx *c, size_t n) {
@@ -283,8 +267,7 @@ int reset_state(cpi_ctx *c) {
     HIPCHK(c, hipMemsetAsync(c->P.epbar, 0, rs * c->nq * (size_t)c->ne, c->stream));
     if (c->P.yielded)
       HIPCHK(c, hipMemsetAsync(c->P.yielded, 0, 8 * (size_t)c->P.ywords * c->nblocks, c->stream));
-    if (c->P.defer)
-      HIPCHK(c, hipMemsetAsync(c->P.defer, 0, sizeof(unsigned) * c->nblocks, c->stream));
+    if (c->P.mlp_ctr) HIPCHK(c, hipMemsetAsync(c->P.mlp_ctr, 0, sizeof(unsigned long long), c->stream));
   }
   HIPCHK(c, hipMemsetAsync(c->P.epart, 0, sizeof(double) * EP_N * c->n_rows, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -321,6 +304,11 @@ int check_flags(cpi_ctx *c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (h.flags & 2u) return fail(c, CPI_ERR_NUMERIC, "inverted or degenerate element (detJ <= 0) during the step");
   if (h.flags & 1u) return fail(c, CPI_ERR_NUMERIC, "non-finite value in the energy terms (NaN/Inf state)");
+  if (h.flags & 4u)
+    return fail(c, CPI_ERR_NUMERIC,
+                "time step " + std::to_string(h.dt) + " above the stable limit " +
+                    std::to_string(std::min(h.row[7], h.dt_contact)) +
+                    " of the configuration (element CFL / wall spring)");
   return CPI_OK;
 }
 
@@ -343,6 +331,7 @@ extern "C" void cpi_default_options(cpi_options *o) {
   o->track_energy = 1;
   o->contact_slave_only = 0;
   o->fixed_bc = 0;
+  o->debug_fill_nan = 0;
   o->block_elements = 0;
   o->device = 0;
   o->gravity[0] = 0.0;
@@ -360,7 +349,6 @@ extern "C" void cpi_destroy(cpi_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graphs(c);
   for (void *p : c->allocs) (void)hipFree(p);
-  if (c->h_plastic) (void)hipHostFree(c->h_plastic);
   mlp_free(c->mlp);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -598,23 +586,24 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       if ((rc = dalloc(c, &yw, (size_t)P.ywords * B.nblocks))) return rc;
       P.yielded = yw;
     }
-    // surrogate mode: the elastic pass's deferred-block flags, and the
-    // host word that tells cpi_step a trial stress has yielded
+    // surrogate mode: the step's counter, the deferred-block list and the
+    // network queue (one row per element at most, plus a tile of padding
+    // the last tile reads)
     if (npe == 4 && c->phys == 2) {
-      unsigned *df = nullptr;
-      if ((rc = dalloc(c, &df, (size_t)B.nblocks))) return rc;
-      P.defer = df;
-      void *hp = nullptr;
-      HIPCHK(c, hipHostMalloc(&hp, sizeof(unsigned), hipHostMallocMapped));
-      c->h_plastic = (unsigned *)hp;
-      *c->h_plastic = 0u;
-      void *dp = nullptr;
-      HIPCHK(c, hipHostGetDevicePointer(&dp, hp, 0));
-      P.plastic_host = (unsigned *)dp;
+      unsigned long long *ctr = nullptr;
+      int2 *lst = nullptr;
+      uint16_t *rows = nullptr;
+      const size_t nrows = (size_t)ne + 16;
+      if ((rc = dalloc(c, &ctr, 1)) || (rc = dalloc(c, &lst, (size_t)B.nblocks)) ||
+          (rc = dalloc(c, &rows, nrows * 16)))
+        return rc;
+      HIPCHK(c, hipMemsetAsync(rows, 0, nrows * 32, c->stream));
+      P.mlp_ctr = ctr;
+      P.mlp_list = lst;
+      P.mlp_rows = rows;
     }
     P.nblocks = B.nblocks;
     c->smem = elem_block_smem(npe, c->prec, c->phys, P.ml, P.max_el, P.kmax);
-    P.mlp_stage_offset = (int)elem_block_mlp_offset(npe, c->prec, P.ml, P.max_el, P.kmax);
     if (c->smem > (size_t)prop.sharedMemPerBlock)
       return fail(c, CPI_ERR_INVALID, "element block needs " + std::to_string(c->smem) +
                                           " bytes of LDS; lower block_elements");
@@ -655,6 +644,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   P.hist = hist;
   P.hist_cap = 4096;
   P.mlp_tol = opt->mlp_tolerance;
+  P.sample_keep = 1ull << 32;  // the recorder keeps every yielding point
   if (c->phys == 2) {
     if ((rc = mlp_alloc(c->mlp, c->nblocks, c->stream, c->err))) return rc;
     P.mlp = c->mlp.d_params;
@@ -739,19 +729,22 @@ extern "C" int cpi_init(cpi_ctx *c, double dt) {
   int rc = reset_state(c);
   if (!rc) rc = compute_mass(c);
   if (rc) return rc;
+  double dtc = INFINITY;
   if (dt <= 0.0) {
-    double dte, dtc;
+    double dte;
     if ((rc = stable_dt(c, &dte, &dtc))) return rc;
     const double ds = 1.0 / std::sqrt(1.0 / (dte * dte) + 1.0 / (dtc * dtc));
     dt = (dt == 0.0 ? 0.5 : -dt) * ds;
     if (!(dt > 0.0) || !std::isfinite(dt)) return fail(c, CPI_ERR_NUMERIC, "stable dt is not positive");
+  } else if (c->phys != CPI_PHYSICS_COMPAT) {
+    double dte;
+    if ((rc = stable_dt(c, &dte, &dtc))) return rc;
   }
   c->dt = dt;
   c->step = 0;
-  c->P.mlp_all = 0;
-  if (c->h_plastic) *(volatile unsigned *)c->h_plastic = 0u;
   Ctl h{};
   h.dt = dt;
+  h.dt_contact = dtc;
   HIPCHK(c, hipMemcpyAsync(c->P.ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
   hipError_t e = launch_node_init(c->P, c->stream);
   if (e == hipSuccess) e = launch_energy_reduce(c->P, c->stream);
@@ -769,8 +762,7 @@ extern "C" int cpi_step(cpi_ctx *c, int32_t nsteps) {
   if (nsteps < 0) return fail(c, CPI_ERR_INVALID, "nsteps < 0");
   for (int i = 0; i < nsteps;) {
     const bool aux = (i == nsteps - 1);
-    mlp_mode_update(c);
-    const auto &G = c->gs[c->P.mlp_all];
+    const auto &G = c->gs;
     if (G.plain) {
       int k = 0;
       while (k < 2 && !(G.run[k] && nsteps - 1 - i >= kGraphRuns[k])) k++;
@@ -983,23 +975,25 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
                                    int32_t *count) {
   if (!c || !out || !count) return CPI_ERR_INVALID;
   if (!c->inited) return fail(c, CPI_ERR_STATE, "cpi_profile_kernels before cpi_init");
-  // with the surrogate's elastic pass the element pass is two launches:
-  // every other step times them one by one, "elem_pass" (the elastic pass)
-  // and "mlp_walk" (the deferred-block walk), and the steps between time
-  // the pair with no event between the launches ("elem_block"; an event
-  // there adds a few microseconds)
-  const char *names[4] = {c->scatter == CPI_SCATTER_BLOCK ? "elem_block" : "elem_direct",
+  // with the surrogate the element pass is three launches: every other step
+  // times them one by one, "elem_pass" (the elastic pass), "mlp_queue" (the
+  // network on the queue) and "mlp_walk" (the deferred blocks), and the
+  // steps between time the three with no event between them ("elem_block";
+  // an event there adds a few microseconds)
+  const bool mlp = c->scatter == CPI_SCATTER_BLOCK && c->phys == 2;
+  const char *names[5] = {c->scatter == CPI_SCATTER_BLOCK ? "elem_block" : "elem_direct",
                           c->scatter == CPI_SCATTER_BLOCK ? "node_bnd" : "node_all", "elem_pass",
-                          "mlp_walk"};
-  double tot[4] = {0, 0, 0, 0};
-  int64_t launches[4] = {0, 0, 0, 0};
-  hipEvent_t ev[4];
+                          "mlp_queue", "mlp_walk"};
+  double tot[5] = {0, 0, 0, 0, 0};
+  int64_t launches[5] = {0, 0, 0, 0, 0};
+  // ev[0] step start, ev[1] element pass end, ev[2] node pass end,
+  // ev[3], ev[4] after the first and second surrogate launch
+  hipEvent_t ev[5];
   for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
   for (int i = 0; i < nsteps; i++) {
-    mlp_mode_update(c);
     Params p = c->P;
     p.store_aux = 0;
-    const bool split = c->scatter == CPI_SCATTER_BLOCK && c->h_plastic && !p.mlp_all && i % 2 == 0;
+    const bool split = mlp && i % 2 == 0;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
     hipError_t e = hipSuccess;
     if (split) {
@@ -1009,8 +1003,11 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
       if (e == hipSuccess)
         e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream,
                               2);
-      launches[2] += 1;
-      launches[3] += 1;
+      HIPCHK(c, hipEventRecord(ev[4], c->stream));
+      if (e == hipSuccess)
+        e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream,
+                              3);
+      for (int k = 2; k < 5; k++) launches[k] += 1;
     } else if (c->scatter == CPI_SCATTER_BLOCK) {
       p.epart_base = 0;
       e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream);
@@ -1041,22 +1038,24 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
     if (!split) tot[0] += t0;
     tot[1] += t1;
     if (split) {
-      float ta = 0, tb = 0;
+      float ta = 0, tb = 0, tc = 0;
       HIPCHK(c, hipEventElapsedTime(&ta, ev[0], ev[3]));
-      HIPCHK(c, hipEventElapsedTime(&tb, ev[3], ev[1]));
+      HIPCHK(c, hipEventElapsedTime(&tb, ev[3], ev[4]));
+      HIPCHK(c, hipEventElapsedTime(&tc, ev[4], ev[1]));
       tot[2] += ta;
       tot[3] += tb;
+      tot[4] += tc;
     }
   }
   for (auto &e : ev) (void)hipEventDestroy(e);
   c->step += nsteps;
-  // one step only: the element pass is the sum of its two launches
+  // one step only: the element pass is the sum of its launches
   if (launches[0] == 0 && launches[2] > 0) {
-    tot[0] = tot[2] + tot[3];
+    tot[0] = tot[2] + tot[3] + tot[4];
     launches[0] = launches[2];
   }
   // the split entries only when some step was split
-  int n = std::min<int>(cap, launches[2] > 0 ? 4 : 2);
+  int n = std::min<int>(cap, launches[2] > 0 ? 5 : 2);
   for (int k = 0; k < n; k++) {
     std::memset(out[k].name, 0, sizeof(out[k].name));
     std::strncpy(out[k].name, names[k], sizeof(out[k].name) - 1);
@@ -1165,6 +1164,7 @@ extern "C" int cpi_set_surrogate(cpi_ctx *c, const cpi_mlp *m) {
   HIPCHK(c, hipStreamSynchronize(m->stream));
   HIPCHK(c, hipMemcpyAsync(c->mlp.d_weights, m->W, sizeof(MlpWeights), hipMemcpyDeviceToDevice, c->stream));
   MlpParams h{};
+  h.weights = c->mlp.d_weights;
   h.w1 = c->mlp.d_weights->w1;
   h.w2p = c->mlp.d_weights->w2p;
   h.w3p = c->mlp.d_weights->w3p;
@@ -1229,6 +1229,13 @@ extern "C" int cpi_samples_enable(cpi_ctx *c, int32_t cap) {
   c->P.samples = c->mlp.d_samples;
   c->P.sample_count = c->mlp.d_sample_count;
   c->P.sample_cap = c->mlp.sample_cap;
+  return refresh_graphs(c);
+}
+
+extern "C" int cpi_samples_subsample(cpi_ctx *c, double keep, uint64_t seed) {
+  if (!c || !(keep >= 0.0 && keep <= 1.0)) return CPI_ERR_INVALID;
+  c->P.sample_keep = keep >= 1.0 ? (1ull << 32) : (unsigned long long)(keep * 4294967296.0);
+  c->P.sample_seed = seed;
   return refresh_graphs(c);
 }
 

@@ -27,10 +27,12 @@ struct Ctl {
   double dt;
   int64_t step;    // completed loop iterations
   int64_t nrows;   // energy rows written to the history ring
-  uint32_t flags;  // bit 0: non-finite energy, bit 1: inverted element
+  uint32_t flags;  // bit 0: non-finite energy, bit 1: inverted element,
+                   // bit 2: dt above the stable limit of the configuration
   uint32_t pad;
   double e0;       // KE + Epen of the initial state
   double row[8];   // latest energy row (CPI_E_* order)
+  double dt_contact;  // wall-spring limit 2 sqrt(m_min / k) (fixed: mass is)
 };
 
 struct MlpParams;  // surrogate, see mlp.h
@@ -106,20 +108,21 @@ struct Params {
   const MlpParams *mlp;  // null unless the surrogate is enabled
   double mlp_tol;        // fallback threshold on |f_yield| / sy0
   unsigned long long *mlp_stats;  // [nblocks][2] surrogate accepted / fell back
-  // surrogate mode, block scatter: 1 for a block the elastic pass left to
-  // the surrogate kernel this step (set by k_elem_block4, cleared by
-  // k_elem_block4_mlp); [nblocks]
-  unsigned *defer;
-  // mapped pinned host word the elastic pass sets when a trial stress
-  // yields (the host then drops the elastic pass: mlp_all)
-  unsigned *plastic_host;
-  int mlp_all;  // the surrogate kernel takes every block, no elastic pass
-  int nblocks;  // element blocks of the block scatter
-                                  // per block of the element kernel (optional)
+  // surrogate mode (block scatter): the elastic pass defers a block with a
+  // yielding element, lists it and queues the network inputs of its
+  // yielding points; mlp_ctr = (deferred blocks << 32) | queued points of
+  // the step, zeroed by the boundary-node kernel at the end of the step
+  unsigned long long *mlp_ctr;
+  int2 *mlp_list;        // [nblocks] (block, its first queue row)
+  uint16_t *mlp_rows;    // [ne + 16] rows of 32 bytes (see mlp_infer.h)
+  int nblocks;           // element blocks of the block scatter
   float *samples;        // return-mapping sample buffer (optional)
   int *sample_count;
   int sample_cap;
-  int mlp_stage_offset;  // byte offset of the per-wave MLP staging in LDS
+  // a yielding point is recorded when the 32-bit hash of (seed, step,
+  // element, point) is below sample_keep (2^32: every point)
+  unsigned long long sample_keep;
+  unsigned long long sample_seed;
 };
 
 // ---------------------------------------------------------------------------

@@ -92,7 +92,7 @@ __device__ __forceinline__ void load_state_if(const Params &p, int e, int q, boo
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
 // point's internal work and plastic dissipation (times wd) to ea.  (The
-// surrogate's return map runs in its own workgroup phase, k_elem_block4_mlp.)
+// surrogate's return map is applied by the tet4 block kernel's walk.)
 template <typename R, int PHYS, bool PRE = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
@@ -132,7 +132,7 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     }
     const R ep0 = ep;
     const R dg = radial_return<R>(p, s, ep);
-    if (p.samples && dg > R(0)) record_sample<R>(p, de, so, ep0, s, ep);
+    if (p.samples && dg > R(0)) record_sample<R>(p, e, q, de, so, ep0, s, ep);
 #pragma unroll
     for (int c = 0; c < 6; c++) w += s[c] * de[c];
     ea.wint += double(wd * R(0.5) * w);

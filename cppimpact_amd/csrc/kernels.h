@@ -7,16 +7,14 @@
 
 namespace cpi {
 
-size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax);
 size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax);
 int elem_block_threads(int npe, int rule);
 int elem_block_default_elements(int npe);
 int elem_direct_threads(int npe);
 
-// part: 0 = the whole element pass; with the surrogate's elastic pass
-// (PHYS_EP_MLP before the switch to the surrogate kernel alone) 1 launches
-// only the elastic pass and 2 only the deferred-block walk, so each can be
-// timed on its own
+// part: 0 = the whole element pass; with the surrogate (PHYS_EP_MLP) 1
+// launches only the elastic pass, 2 only the network on the queue and 3
+// only the walk over the deferred blocks, so each can be timed on its own
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,
                              int nblocks, size_t smem, hipStream_t s, int part = 0);
 hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem);
@@ -36,6 +34,9 @@ hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s);
 hipError_t launch_stress_elem(const Params &p, double *out, hipStream_t s);
 hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcase, double nu,
                                double *out, hipStream_t s);
+// the surrogate network on queue rows [0, n) (rows hold ceil(n/16)*16 rows
+// of 32 bytes; see mlp_infer.h)
+hipError_t launch_mlp_rows(const MlpParams *P, uint16_t *rows, int n, hipStream_t s);
 // copies n4 32-byte records
 hipError_t launch_stream_copy(const void *src, void *dst, long n4, hipStream_t s);
 

@@ -2,7 +2,8 @@
 //
 // One loop iteration of Dynamics::solve (dynamics.h:471-511) is, with the
 // LDS-assembled scatter (the default):
-//   k_elem_block4 (tet4; k_elem_block4_mlp with the surrogate),
+//   k_elem_block4 (tet4; with the surrogate the elastic pass, k_mlp_queue
+//                 and k_elem_block4_walk),
 //   k_elem_block10 (tet10, 4-point rules), k_elem_block (tet10, 5 points)
 //                 one workgroup per element block: stage the block's nodes
 //                 in LDS, compute element forces (and the stress update),
@@ -367,16 +368,27 @@ k_elem_block10(Params p) {
 // global load of the workgroup (stress state, local connectivity, the
 // contribution lists, node records, slots) is issued before the first
 // barrier.  LDS: node records (LNodes), corner forces [max_el*4][3] R in
-// jagged-diagonal slots, the slot offsets off[k].  PHYS_EP_MLP is the
-// surrogate mode's elastic pass: a block whose trial stresses all stay
-// elastic finishes here, one with a yielding element is deferred whole to
-// k_elem_block4_mlp before it writes anything.
-template <typename R, int PHYS, bool AUX, int NT, int WPE>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_elem_block4(Params p) {
+// jagged-diagonal slots, the slot offsets off[k].
+//
+// With the surrogate (PHYS_EP_MLP, FP32) a step is three launches:
+//   elastic pass  this body over every block (WALK = false): a block whose
+//                 trial stresses all stay inside the yield surface finishes
+//                 here; a block with a yielding element writes nothing of
+//                 its state, queues the network inputs of its yielding
+//                 points (in thread order) and lists itself (one 64-bit
+//                 atomic per deferred block: list slot and first queue row)
+//   k_mlp_queue   the network on the whole queue (mlp_infer.h)
+//   walk          this body (WALK = true) over the listed blocks: yielding
+//                 elements take their queued outputs through mlp_apply
+//                 (acceptance checks, FP64 exact fallback)
+// A deferred block computes the same trial stresses, hence the same
+// yielding set and queue order, in both launches, and which workgroup
+// finishes a block never changes its results.
+template <typename R, int PHYS, bool AUX, int NT, bool WALK>
+__device__ __forceinline__ void block4(const Params &p, const int b, const unsigned qbase,
+                                       int *wcnt, unsigned *misc) {
   extern __shared__ double smem[];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   // the node side needs only b: ids, slots and force ranges at a fixed
   // stride, then the node records
   int g = -1, slot = -1, deg = 0;
@@ -398,11 +410,15 @@ k_elem_block4(Params p) {
   const int e = es + t;
   uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
   PreState<R> pre;
-  // yielded bits of this wave's elements
-  const long yi = (long)b * p.ywords + __builtin_amdgcn_readfirstlane(t >> 6);
+  // yielded bits of this wave's elements: a block has ywords words, one
+  // per 64 elements of the largest block, and a wave past them (a block
+  // plan of at most 320 elements per block) has no elements and no word
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const bool has_word = p.yielded && wv < p.ywords;
+  const long yi = (long)b * p.ywords + wv;
   unsigned long long yw = ~0ull;
   if constexpr (PHYS != PHYS_COMPAT)
-    if (p.yielded) yw = p.yielded[yi];
+    if (has_word) yw = p.yielded[yi];
   if (has_e) {
     lc4 = *(const uint2 *)(p.conn_local + 4L * e);
     ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
@@ -411,29 +427,29 @@ k_elem_block4(Params p) {
   const bool has_n = g >= 0;
   if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
   if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
+  if constexpr (WALK)
+    if (t == 0) misc[0] = misc[1] = 0u;
   __syncthreads();
 
   EAcc ea;
-  [[maybe_unused]] R snew[6];
+  int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
+              (int)(lc4.y >> 16)};
+  int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
+               (int)(ps4.y >> 16)};
+  auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
+  auto put = [&](int k, const R fk[3]) {
+    R *dst = lf + 3 * ps[k];
+#pragma unroll
+    for (int c = 0; c < 3; c++) dst[c] = fk[c];
+  };
+  [[maybe_unused]] R Ji[9], de[6], wd, snew[6], wk;
   [[maybe_unused]] bool yielding = false;
   if (has_e) {
-    const int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
-                      (int)(lc4.y >> 16)};
-    const int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
-                       (int)(ps4.y >> 16)};
-    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
-    auto put = [&](int k, const R fk[3]) {
-      R *dst = lf + 3 * ps[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) dst[c] = fk[c];
-    };
     if constexpr (PHYS == PHYS_EP_MLP) {
-      // the elastic pass of the surrogate mode: an element whose trial
-      // stress stays inside the yield surface finishes as in the surrogate
-      // kernel's first phase; a yielding one sends the whole block there
-      R Ji[9], de[6], wd;
+      // an element whose trial stress stays inside the yield surface
+      // finishes here; a yielding one waits for the network
       tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, ea);
-      R wk = ep_trial<R>(p, de, pre.s, snew);
+      wk = ep_trial<R>(p, de, pre.s, snew);
       if (!(yield_overshoot<R>(p, snew, pre.ep) > R(0))) {
 #pragma unroll
         for (int c = 0; c < 6; c++) wk += snew[c] * de[c];
@@ -448,189 +464,73 @@ k_elem_block4(Params p) {
   }
   if constexpr (PHYS == PHYS_EP) {
     const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
-    if (p.yielded && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
+    if (has_word && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
   if constexpr (PHYS == PHYS_EP_MLP) {
-    // nothing of the block has reached global memory yet: a block with a
-    // yielding element goes whole to k_elem_block4_mlp, and the host is
-    // told (it then launches the surrogate kernel alone over every block)
-    if (__syncthreads_or(yielding)) {
-      if (t == 0) {
-        p.defer[b] = 1u;
-        if (p.plastic_host) *(volatile unsigned *)p.plastic_host = 1u;
-      }
-      return;
-    }
-    if (has_e) store_state<R>(p, e, 0, snew, pre.ep, false);
-  } else {
+    // rank of each yielding element in thread order
+    const uint64_t m = __ballot(yielding);
+    if (lane == 0) wcnt[w] = __popcll(m);
     __syncthreads();
-  }
-
-  if (has_n) {
-    double f[3];
-    node_force_sum<R>(lf, joff, t, deg, f);
-    if (slot < 0) {
-      double x[3], v[3];
-      ln.get(t, x, v);
-      node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
-    } else {
-      store_partial<R>(p, slot, f);
+    int total = 0, off = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+      const int c = wcnt[i];
+      off += i < w ? c : 0;
+      total += c;
     }
-  }
-  wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
-}
-
-// tet4, FP32, J2 with the MLP surrogate of the return map.  Same structure
-// as k_elem_block4, with the return map taken out of the per-thread element
-// code so the network's registers never add to the element's:
-//   phase 1  every element: kinematics and trial stress; elastic ones
-//            finish here exactly as in k_elem_block4;
-//   queue    yielding elements in thread order (per-wave ballots + offsets:
-//            deterministic);
-//   phase 2  only if the queue is not empty: the waves take the queue 64 at a
-//            time, rebuild each element's kinematics from the LDS node
-//            records, evaluate the network with the matrix cores (samples
-//            packed, see mlp_wave_infer), check the yield residual (exact
-//            return on failure), then the element's work, state and forces.
-// LDS: node records, corner forces, one MLP staging area per wave, the
-// queue and the per-wave counts.
-template <bool AUX, int NT>
-__device__ __forceinline__ void mlp_block(const Params &p, const int b, uint16_t *queue, int *wcnt,
-                                          unsigned *mcnt) {
-  using R = float;
-  extern __shared__ double smem[];
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  int g = -1, slot = -1, deg = 0;
-  if (t < p.ml) {
-    const long o = (long)b * p.ml + t;
-    g = p.nid_pad[o];
-    slot = p.slot_pad[o];
-    deg = p.deg_pad[o];
-  }
-  const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
-  const int32_t *mt = p.meta + 8 * b;
-  const int es = mt[0], nel = mt[1];
-  const LNodes ln{smem, p.ml};
-  R *lf = (R *)((char *)smem + lds_node_bytes(p.ml));
-  uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 4, p.max_el));
-  const double dt = p.ctl->dt;
-
-  const bool has_e = t < nel;
-  const int e = es + t;
-  uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
-  PreState<R> pre;
-  if (has_e) {
-    lc4 = *(const uint2 *)(p.conn_local + 4L * e);
-    ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
-    load_state<R>(p, e, 0, pre.s, pre.ep);
-  }
-  const bool has_n = g >= 0;
-  if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
-  if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
-  if (t == 0) mcnt[0] = mcnt[1] = 0u;
-  __syncthreads();
-
-  auto unpack4 = [](uint2 v, int o[4]) {
-    o[0] = (int)(v.x & 0xffffu);
-    o[1] = (int)(v.x >> 16);
-    o[2] = (int)(v.y & 0xffffu);
-    o[3] = (int)(v.y >> 16);
-  };
-  EAcc ea;
-  bool yielding = false;
-  if (has_e) {
-    int l[4], ps[4];
-    unpack4(lc4, l);
-    unpack4(ps4, ps);
-    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
-    auto put = [&](int k, const R fk[3]) {
-      R *dst = lf + 3 * ps[k];
-#pragma unroll
-      for (int c = 0; c < 3; c++) dst[c] = fk[c];
-    };
-    R Ji[9], de[6], wd, s[6];
-    tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, ea);
-    R wk = ep_trial<R>(p, de, pre.s, s);
-    if (!(yield_overshoot<R>(p, s, pre.ep) > R(0))) {
-#pragma unroll
-      for (int c = 0; c < 6; c++) wk += s[c] * de[c];
-      ea.wint += double(wd * R(0.5) * wk);
-      store_state<R>(p, e, 0, s, pre.ep, false);
-      tet4_forces<R>(Ji, s, wd, put);
-    } else {
-      yielding = true;
-    }
-  }
-  const uint64_t m = __ballot(yielding);
-  if (lane == 0) wcnt[w] = __popcll(m);
-  __syncthreads();
-  int total = 0, off = 0;
-#pragma unroll
-  for (int i = 0; i < NT / 64; i++) {
-    const int c = wcnt[i];
-    off += i < w ? c : 0;
-    total += c;
-  }
-  if (total > 0) {
-    if (yielding) queue[off + __popcll(m & ((1ull << lane) - 1ull))] = (uint16_t)t;
-    __syncthreads();
-    for (int base = w * 64; base < total; base += NT) {
-      const int k = base + lane;
-      const bool act = k < total;
-      const int e2 = es + queue[act ? k : base];
-      const uint2 lc2 = *(const uint2 *)(p.conn_local + 4L * e2);
-      auto get = [&](int kk, double xk[3], double vk[3]) {
-        int l[4];
-        unpack4(lc2, l);
-        ln.get(l[kk], xk, vk);
-      };
-      // inputs, then the network: nothing but the outputs, ep and the
-      // element's indices stays live across the matrix-core code
-      float y[kMlpOut];
-      R ep;
-      {
-        R so[6], Ji[9], de[6], wd;
-        load_state<R>(p, e2, 0, so, ep);
-        EAcc again;  // counted in phase 1
-        tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, again);
-        float x[kMlpIn];
-        mlp_inputs<R>(*p.mlp, de, so, ep, x);
-        mlp_wave_infer(*p.mlp, x, y, mlp_stage(p), act, __ballot(act));
+    const int rank = off + __popcll(m & ((1ull << lane) - 1ull));
+    if constexpr (!WALK) {
+      if (total > 0) {
+        // nothing of the block has reached global memory yet: list it and
+        // queue its yielding points' network inputs
+        if (t == 0) {
+          const unsigned long long old =
+              atomicAdd(p.mlp_ctr, (1ull << 32) | (unsigned long long)total);
+          p.mlp_list[old >> 32] = make_int2(b, (int)(unsigned)old);
+          misc[0] = (unsigned)old;
+        }
+        __syncthreads();
+        if (yielding) {
+          float x[kMlpIn];
+          mlp_inputs<R>(*p.mlp, de, pre.s, pre.ep, x);
+          mlp_store_inputs(p.mlp_rows + (size_t)(misc[0] + rank) * kMlpRowHalves, x);
+        }
+        return;
       }
-      // the element again from LDS and the state (cached), then the return
-      R so[6], Ji[9], de[6], wd, s[6];
-      load_state<R>(p, e2, 0, so, ep);
-      EAcc again;
-      tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, again);
-      R wk = ep_trial<R>(p, de, so, s);
+      if (has_e) store_state<R>(p, e, 0, snew, pre.ep, false);
+    } else {
       int how = MLP_ELASTIC;
-      const R q = yield_overshoot<R>(p, s, ep);
-      R dg = R(0);
-      if (act && q > R(0)) dg = mlp_apply<R>(p, y, q, ep, s, how);
-      mlp_count(mcnt, how == MLP_ACCEPTED, how == MLP_FALLBACK);
-      if (act) {
+      if (has_e && !yielding) store_state<R>(p, e, 0, snew, pre.ep, false);
+      if (yielding) {
+        // the element again from the LDS node records (the same bits as
+        // above; counted there), so nothing of it stays live across the
+        // return map
+        R Jy[9], dy[6], wy, sy[6];
+        EAcc again;
+        tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Jy, dy, wy, again);
+        R wky = ep_trial<R>(p, dy, pre.s, sy);
+        R ep = pre.ep;
+        const R q = yield_overshoot<R>(p, sy, ep);
+        float y[kMlpOut];
+        mlp_load_outputs(p.mlp_rows + (size_t)(qbase + rank) * kMlpRowHalves, y);
+        const R dg = mlp_apply<R>(p, y, q, ep, sy, how);
 #pragma unroll
-        for (int c = 0; c < 6; c++) wk += s[c] * de[c];
-        ea.wint += double(wd * R(0.5) * wk);
+        for (int c = 0; c < 6; c++) wky += sy[c] * dy[c];
+        ea.wint += double(wy * R(0.5) * wky);
         if (dg > R(0))
-          ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-        store_state<R>(p, e2, 0, s, ep);
-        const uint2 ps2 = *(const uint2 *)(p.csr_pos + 4L * e2);
-        auto put = [&](int kk, const R fk[3]) {
-          int ps[4];
-          unpack4(ps2, ps);
-          R *dst = lf + 3 * ps[kk];
-#pragma unroll
-          for (int c = 0; c < 3; c++) dst[c] = fk[c];
-        };
-        tet4_forces<R>(Ji, s, wd, put);
+          ea.wpl += double(wy * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
+        store_state<R>(p, e, 0, sy, ep);
+        tet4_forces<R>(Jy, sy, wy, put);
+      }
+      mlp_count(misc, how == MLP_ACCEPTED, how == MLP_FALLBACK);
+      __syncthreads();
+      if (t == 0 && p.mlp_stats && (misc[0] | misc[1])) {
+        p.mlp_stats[2L * b] += misc[0];
+        p.mlp_stats[2L * b + 1] += misc[1];
       }
     }
+  } else {
     __syncthreads();
-    if (t == 0 && p.mlp_stats && (mcnt[0] | mcnt[1])) {
-      p.mlp_stats[2L * b] += mcnt[0];
-      p.mlp_stats[2L * b + 1] += mcnt[1];
-    }
   }
 
   if (has_n) {
@@ -647,38 +547,47 @@ __device__ __forceinline__ void mlp_block(const Params &p, const int b, uint16_t
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
 }
 
-// LIST = false: one block per workgroup, every block (the surrogate kernel
-// alone does the step's element pass).  LIST = true: the blocks the elastic
-// pass (k_elem_block4 with PHYS_EP_MLP) deferred this step; workgroup i of
-// a small grid reads the flags of blocks i, i + grid, ... (K <= 64 of them)
-// in one wave, takes the flagged ones in order and clears their flags
-// (until the first yield none is, and the launch only reads flags; the
-// stride spreads a cluster of yielding blocks over the workgroups).
-template <bool AUX, int NT, int WPE, bool LIST>
+template <typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_elem_block4_mlp(Params p) {
-  __shared__ uint16_t queue[NT];
+k_elem_block4(Params p) {
   __shared__ int wcnt[NT / 64];
-  __shared__ unsigned mcnt[2];  // network accepted / fell back
-  if constexpr (!LIST) {
-    mlp_block<AUX, NT>(p, xcd_chunked(blockIdx.x, gridDim.x), queue, wcnt, mcnt);
-  } else {
-    // every wave reads the same flags, so all agree on the list without a
-    // barrier before the first block
-    const int K = (p.nblocks + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int lane = (int)(threadIdx.x & 63);
-    const int bl = (int)blockIdx.x + lane * (int)gridDim.x;
-    unsigned long long m = __ballot(lane < K && bl < p.nblocks && p.defer[bl] != 0u);
-    while (m) {
-      const int b = (int)blockIdx.x + (__ffsll((long long)m) - 1) * (int)gridDim.x;
-      m &= m - 1ull;
-      mlp_block<AUX, NT>(p, b, queue, wcnt, mcnt);
-      // every thread is past the block's last LDS read before the next
-      // block's staging
-      __syncthreads();
-      if (threadIdx.x == 0) p.defer[b] = 0u;
-    }
-  }
+  __shared__ unsigned misc[2];
+  block4<R, PHYS, AUX, NT, false>(p, xcd_remap(blockIdx.x, gridDim.x), 0u, wcnt, misc);
+}
+
+// The surrogate step's third launch: one workgroup per block of the grid,
+// workgroup i finishing the i-th deferred block listed this step, if there
+// is one, so the hardware spreads a plastic zone over the CUs as it would
+// the blocks of an element pass (the list order depends on atomic order,
+// the results of a block do not).  A loop over several listed blocks per
+// workgroup holds the kernel arguments in SGPRs across iterations and
+// spills 40-50 VGPRs.
+template <bool AUX, int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_elem_block4_walk(Params p) {
+  __shared__ int wcnt[NT / 64];
+  __shared__ unsigned misc[2];
+  if (blockIdx.x >= (unsigned)(*p.mlp_ctr >> 32)) return;
+  const int2 ent = p.mlp_list[blockIdx.x];
+  block4<float, PHYS_EP_MLP, AUX, NT, true>(p, ent.x, (unsigned)ent.y, wcnt, misc);
+}
+
+// The network on queue rows [0, n): n from the step's counter (ctr) or
+// given.  Each workgroup with a tile to do copies the weights into LDS;
+// its waves take tiles w, w + waves, ...
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mlp_queue(const MlpParams *P, uint16_t *rows,
+                                                  const unsigned long long *ctr, int n_given) {
+  __shared__ __attribute__((aligned(16))) MlpWeights W;
+  constexpr int WPB = NT / 64;
+  const unsigned n = ctr ? (unsigned)(*ctr & 0xffffffffull) : (unsigned)n_given;
+  const int ntiles = (int)((n + 15u) >> 4);
+  if ((int)blockIdx.x * WPB >= ntiles) return;
+  mlp_lds_load(P->weights, &W);
+  __syncthreads();
+  for (int tile = (int)blockIdx.x * WPB + (int)(threadIdx.x >> 6); tile < ntiles;
+       tile += (int)gridDim.x * WPB)
+    mlp_tile(W, rows + (size_t)tile * 16 * kMlpRowHalves);
 }
 
 template <bool AUX, typename R>
@@ -719,7 +628,12 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
   wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea, row);
-  if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
+  // the last launch of the step: the surrogate's queue and block list start
+  // empty in the next one
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    p.ctl->step += 1;
+    if (p.mlp_ctr) *p.mlp_ctr = 0ull;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -833,8 +747,13 @@ __global__ void __launch_bounds__(256) k_energy_reduce1(Params p) {
   double v[EP_N] = {0, 0, 1e300, 0, 0, 0, 0, 0};
   for (int r = r0 + threadIdx.x; r < r1; r += 256) {
     const double *src = p.epart + (long)r * EP_N;
+    // a stable-dt entry of 0 is a row no kernel has written since the rows
+    // were cleared (cpi_init's own reduction, rows only the half-kick
+    // writes); it takes no part in the minimum.  A degenerate element
+    // (limit 0) is caught by the inverted-element count instead.
 #pragma unroll
-    for (int k = 0; k < EP_N; k++) v[k] = (k == EP_MINDT) ? fmin(v[k], src[k]) : v[k] + src[k];
+    for (int k = 0; k < EP_N; k++)
+      v[k] = (k == EP_MINDT) ? fmin(v[k], src[k] > 0.0 ? src[k] : 1e300) : v[k] + src[k];
   }
 #pragma unroll
   for (int k = 0; k < EP_N; k++) red[threadIdx.x * EP_N + k] = v[k];
@@ -879,6 +798,10 @@ __global__ void __launch_bounds__(64) k_energy_reduce2(Params p) {
     c->row[7] = row[7];
     if (!finite) c->flags |= 1u;
     if (s[EP_BAD] > 0.0) c->flags |= 2u;
+    // the elastoplastic kernels reduce the element CFL limit of the
+    // configuration each step started from (the others leave 1e300): a
+    // fixed dt above it, or above the wall spring's limit, is unstable
+    if (row[7] < 1e300 && !(c->dt <= fmin(row[7], c->dt_contact))) c->flags |= 4u;
     if (p.hist && p.hist_cap > 0) {
       double *h = p.hist + (c->nrows % p.hist_cap) * 8;
       for (int k = 0; k < 8; k++) h[k] = row[k];
@@ -1189,30 +1112,26 @@ constexpr int block_elements(int npe) { return npe == 4 ? 384 : 128; }
 template <typename R> constexpr int tet10_waves() { return 4; }
 
 // minimum waves per SIMD the tet4 element kernels are compiled for: FP64
-// fits 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU).
-// The surrogate kernel's LDS (MLP staging) allows four workgroups per CU,
-// i.e. six waves per SIMD, so it may use 80 VGPRs: with them its spill code
-// stays inside the network phase, off the elastic path.
-// workgroups of the surrogate kernel that walk the blocks the elastic pass
-// deferred: one per CU
-constexpr int kMlpListGrid = 256;
-
+// fits 96 VGPRs (three 6-wave workgroups per CU), FP32 64 (five per CU);
+// the surrogate step's elastic pass has the plain FP32 kernel's structure
+// and budget.  Its walk (network outputs, acceptance checks, the FP64
+// fallback) spills 19 VGPRs at 64 and none at 80: six waves per SIMD, four
+// workgroups per CU.
 template <typename R, int PHYS> constexpr int tet4_waves() {
-  return PHYS == PHYS_EP_MLP ? 6 : std::is_same<R, double>::value ? 5 : 8;
+  return std::is_same<R, double>::value ? 5 : 8;
 }
+constexpr int kMlpWalkWaves = 6;
 
-// LDS of the workgroup element kernels: node records, corner forces and,
-// with the surrogate, one MLP staging area per wave
-size_t elem_block_mlp_offset(int npe, int prec, int ml, int max_el, int kmax) {
+// workgroups of k_mlp_queue in the step (four waves each, every wave
+// looping over tiles)
+constexpr int kMlpQueueGrid = 1024;
+constexpr int kMlpQueueThreads = 256;
+
+// LDS of the workgroup element kernels: node records, corner forces, the
+// slot offsets
+size_t elem_block_smem(int npe, int prec, int, int ml, int max_el, int kmax) {
   const int rsz = prec == 1 ? 4 : 8;
   return (size_t)lds_node_bytes(ml) + lds_force_bytes(rsz, npe, max_el) + lds_off_bytes(kmax);
-}
-
-size_t elem_block_smem(int npe, int prec, int phys, int ml, int max_el, int kmax) {
-  size_t s = elem_block_mlp_offset(npe, prec, ml, max_el, kmax);
-  // the surrogate runs on tet4 only
-  if (phys == PHYS_EP_MLP) s += (size_t)kMlpStageBytes * (block_threads<4, RULE_CENTROID>() / 64);
-  return s;
 }
 
 int elem_block_threads(int npe, int rule) {
@@ -1237,28 +1156,29 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
                                int part) {
   constexpr int NT = block_threads<NPE, RULE>();
   if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
-    // the elastic pass at the plain FP32 kernel's occupancy (no network
-    // staging in its LDS), then the surrogate kernel over the blocks it
-    // deferred (all of them once any trial stress has yielded)
-    constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
-    const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
-    if (p.mlp_all) {
-      if (aux) CPI_LAUNCH((k_elem_block4_mlp<true, NT, W, false>), nblocks, NT, smem, s, p);
-      CPI_LAUNCH((k_elem_block4_mlp<false, NT, W, false>), nblocks, NT, smem, s, p);
-    }
-    // each workgroup of the list walk covers at most 64 blocks
-    const int grid_list = std::max(std::min(nblocks, kMlpListGrid), (nblocks + 63) / 64);
-    if (part != 2) {
+    // the elastic pass, the network on the queue, the walk over the
+    // deferred blocks (part 1, 2, 3 alone for per-launch timing)
+    constexpr int W = tet4_waves<R, PHYS>();
+    if (part == 0 || part == 1) {
       if (aux)
-        hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+        hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
       else
-        hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, WA>), dim3(nblocks), dim3(NT), smem_a, s, p);
+        hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
     }
-    if (part != 1) {
+    if (part == 0 || part == 2) {
+      const int tiles = (p.ne + 15) / 16;
+      const int grid = std::max(1, std::min(kMlpQueueGrid, (tiles + 3) / 4));
+      hipLaunchKernelGGL((k_mlp_queue<kMlpQueueThreads>), dim3(grid), dim3(kMlpQueueThreads), 0, s,
+                         p.mlp, p.mlp_rows, (const unsigned long long *)p.mlp_ctr, 0);
+    }
+    if (part == 0 || part == 3) {
+      const int grid = nblocks;
       if (aux)
-        hipLaunchKernelGGL((k_elem_block4_mlp<true, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
+        hipLaunchKernelGGL((k_elem_block4_walk<true, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
+                           smem, s, p);
       else
-        hipLaunchKernelGGL((k_elem_block4_mlp<false, NT, W, true>), dim3(grid_list), dim3(NT), smem, s, p);
+        hipLaunchKernelGGL((k_elem_block4_walk<false, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
+                           smem, s, p);
     }
     return hipGetLastError();
   } else if constexpr (NPE == 4) {
@@ -1335,15 +1255,12 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
     } else {
       constexpr int NT = block_threads<NPE, RULE>();
       if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
-        constexpr int WA = tet4_waves<R, PHYS_EP>(), W = tet4_waves<R, PHYS>();
-        const size_t smem_a = smem - (size_t)kMlpStageBytes * (NT / 64);
-        hipError_t e = lds_opt_in(k_elem_block4_mlp<true, NT, W, false>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W, false>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<true, NT, W, true>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_mlp<false, NT, W, true>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, WA>, smem_a);
+        constexpr int W = tet4_waves<R, PHYS>();
+        hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<true, NT, kMlpWalkWaves>, smem);
         if (e != hipSuccess) return e;
-        return lds_opt_in(k_elem_block4<R, PHYS, false, NT, WA>, smem_a);
+        return lds_opt_in(k_elem_block4_walk<false, NT, kMlpWalkWaves>, smem);
       } else if constexpr (NPE == 4) {
         constexpr int W = tet4_waves<R, PHYS>();
         hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
@@ -1485,6 +1402,14 @@ hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcas
     default: CPI_DS(10, RULE_REF5);
   }
 #undef CPI_DS
+}
+
+hipError_t launch_mlp_rows(const MlpParams *P, uint16_t *rows, int n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int tiles = (n + 15) / 16;
+  const int grid = std::min(kMlpQueueGrid, (tiles + 3) / 4);
+  CPI_LAUNCH((k_mlp_queue<kMlpQueueThreads>), grid, kMlpQueueThreads, 0, s, P, rows,
+             (const unsigned long long *)nullptr, n);
 }
 
 hipError_t launch_stream_copy(const void *src, void *dst, long n4, hipStream_t s) {

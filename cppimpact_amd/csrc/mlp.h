@@ -56,6 +56,7 @@ struct MlpTrainWeights {
 
 // Weights as the kernels read them.
 struct MlpParams {
+  const MlpWeights *weights;  // the whole image (copied into LDS per workgroup)
   const uint16_t *w1;   // [64][32]  W1[u][k], k >= 13 zero (K = input feature)
   const uint16_t *w2p;  // [64][64]  W2 with K permuted as described above
   const uint16_t *w3p;  // [16][64]  W3 (rows >= 7 zero), K permuted
@@ -66,23 +67,37 @@ struct MlpParams {
   float c11, c12, c44, sy0, H;
 };
 
-// round-to-nearest-even float -> bf16 bits
-__device__ __forceinline__ uint16_t f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// two floats -> packed bf16 bits (a low), round to nearest even: one
+// v_cvt_pk_bf16_f32 on gfx950 instead of a handful of integer operations
+// per value
+__device__ __forceinline__ uint32_t pack_bf2(float a, float b) {
+  const f32x2 v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
+
+// round-to-nearest-even float -> bf16 bits
+__device__ __forceinline__ uint16_t f2bf(float f) { return (uint16_t)pack_bf2(f, 0.f); }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 __device__ __forceinline__ bf16x8 ld_bf8(const uint16_t *p) { return *(const bf16x8 *)p; }
 
+// Makes this wave's LDS writes visible to its own later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ bf16x8 pack_bf8(const float v[8]) {
   union {
-    uint16_t h[8];
+    uint32_t w[4];
     bf16x8 b;
   } u;
 #pragma unroll
-  for (int j = 0; j < 8; j++) u.h[j] = f2bf(v[j]);
+  for (int j = 0; j < 4; j++) u.w[j] = pack_bf2(v[2 * j], v[2 * j + 1]);
   return u.b;
 }
 
@@ -101,9 +116,15 @@ __device__ __forceinline__ f32x4 mfma_tile(const uint16_t *A, int lda, const uin
   return acc;
 }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// z sigmoid(z) with the hardware exponential and reciprocal (1 ulp each): a
+// correctly rounded division would cost a dozen instructions per unit, and
+// the result is rounded to BF16 next anyway
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -114,12 +135,27 @@ __host__ __device__ __forceinline__ int perm_k(int k) {
   return kk * 32 + g * 8 + hi * 4 + j;
 }
 
+// SplitMix64 finaliser: the recorder's subsampling hash
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 // Records (de, sigma_n, ep_n) -> (sigma_{n+1}, ep_{n+1}) for points that
-// yielded.  Slots come from an atomic counter; samples past the capacity are
-// dropped (the counter still counts them).
+// yielded, point q of element e in this step, when the hash of (seed, step,
+// e, q) falls below the keep threshold (a fixed, reproducible subset).
+// Slots come from an atomic counter; samples past the capacity are dropped
+// (the counter still counts them).
 template <typename R>
-__device__ __forceinline__ void record_sample(const Params &p, const R de[6], const R so[6],
-                                              R ep0, const R s[6], R ep1) {
+__device__ __forceinline__ void record_sample(const Params &p, int e, int q, const R de[6],
+                                              const R so[6], R ep0, const R s[6], R ep1) {
+  if (p.sample_keep < (1ull << 32)) {
+    const uint64_t step = (uint64_t)p.ctl->step;
+    const uint64_t h = mix64(p.sample_seed + 0x9E3779B97F4A7C15ull * (step * 8 + q + 1) +
+                             (uint64_t)(uint32_t)e * 0xD1B54A32D192ED03ull);
+    if ((h >> 32) >= p.sample_keep) return;
+  }
   int slot = atomicAdd(p.sample_count, 1);
   if (slot >= p.sample_cap) return;
   float *d = p.samples + (long)slot * kSampleFloats;

@@ -16,6 +16,8 @@
 #include <vector>
 
 #include "../../include/cppimpact_hip.h"
+#include "elem.h"
+#include "kernels.h"
 #include "mlp_host.h"
 #include "mlp_infer.h"
 
@@ -292,55 +294,62 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float *theta, float *am, float
   }
 }
 
-// network outputs through the inference path (one wave per 64 samples)
-__global__ void __launch_bounds__(64) k_mlp_forward(const MlpParams *P, const float *X, int n,
-                                                    float *Y) {
-  __shared__ __attribute__((aligned(16))) uint16_t stage[kMlpStageBytes / 2];
-  const int s = blockIdx.x * 64 + threadIdx.x;
-  float x[kMlpIn], y[kMlpOut];
+// standardized FP32 inputs [n][16] -> BF16 queue rows (the layout the
+// element pass writes)
+__global__ void __launch_bounds__(256) k_mlp_pack_rows(const float *X, int n, uint16_t *rows) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float x[kMlpIn];
 #pragma unroll
-  for (int k = 0; k < kMlpIn; k++) x[k] = s < n ? X[(long)s * 16 + k] : 0.f;
-  mlp_wave_infer(*P, x, y, stage, s < n, __ballot(s < n));
-  if (s < n)
-#pragma unroll
-    for (int o = 0; o < kMlpOut; o++) Y[(long)s * 8 + o] = y[o];
+  for (int k = 0; k < kMlpIn; k++) x[k] = X[(long)i * 16 + k];
+  mlp_store_inputs(rows + (size_t)i * kMlpRowHalves, x);
 }
 
-// the element kernels' return map on given points (dynamic LDS: one stage)
-__global__ void __launch_bounds__(64) k_mlp_return_points(Params p, int n, const float *de,
-                                                          const float *so, const float *ep,
-                                                          float *s_out, float *ep_out, int *how) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const bool act = i < n;
-  const long k = act ? i : 0;
-  float d[6], o[6], s[6];
-  float e = ep[k];
+// The return map of the surrogate step on given points, in the step's
+// three stages: the trial stress and the network inputs of every point into
+// its queue row, the network on the rows, then the surrogate return (with
+// its acceptance checks and FP64 fallback) where the trial stress yields.
+__device__ __forceinline__ float rm_trial(const Params &p, int i, const float *de, const float *so,
+                                          float d[6], float o[6], float s[6]) {
 #pragma unroll
   for (int c = 0; c < 6; c++) {
-    d[c] = de[k * 6 + c];
-    o[c] = so[k * 6 + c];
+    d[c] = de[(long)i * 6 + c];
+    o[c] = so[(long)i * 6 + c];
   }
-  const float c11 = float(p.c11), c12 = float(p.c12), c44 = float(p.c44);
-  s[0] = o[0] + (c11 * d[0] + c12 * (d[1] + d[2]));
-  s[1] = o[1] + (c11 * d[1] + c12 * (d[0] + d[2]));
-  s[2] = o[2] + (c11 * d[2] + c12 * (d[0] + d[1]));
-  s[3] = o[3] + c44 * d[3];
-  s[4] = o[4] + c44 * d[4];
-  s[5] = o[5] + c44 * d[5];
+  return ep_trial<float>(p, d, o, s);
+}
+
+__global__ void __launch_bounds__(256) k_rm_prepare(Params p, int n, const float *de,
+                                                    const float *so, const float *ep,
+                                                    uint16_t *rows) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float d[6], o[6], s[6], x[kMlpIn];
+  (void)rm_trial(p, i, de, so, d, o, s);
+  mlp_inputs<float>(*p.mlp, d, o, ep[i], x);
+  mlp_store_inputs(rows + (size_t)i * kMlpRowHalves, x);
+}
+
+__global__ void __launch_bounds__(256) k_rm_finish(Params p, int n, const float *de,
+                                                   const float *so, const float *ep,
+                                                   const uint16_t *rows, float *s_out,
+                                                   float *ep_out, int *how) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float d[6], o[6], s[6];
+  (void)rm_trial(p, i, de, so, d, o, s);
+  float e = ep[i];
   const float q = yield_overshoot<float>(p, s, e);
-  const bool yielding = act && q > 0.f;
-  float x[kMlpIn], y[kMlpOut];
-  mlp_inputs<float>(*p.mlp, d, o, e, x);
-  const uint64_t mask = __ballot(yielding);
-  if (mask) mlp_wave_infer(*p.mlp, x, y, mlp_stage(p), yielding, mask);
   int h = MLP_ELASTIC;
-  if (yielding) mlp_apply<float>(p, y, q, e, s, h);
-  if (act) {
-#pragma unroll
-    for (int c = 0; c < 6; c++) s_out[k * 6 + c] = s[c];
-    ep_out[k] = e;
-    how[k] = h;
+  if (q > 0.f) {
+    float y[kMlpOut];
+    mlp_load_outputs(rows + (size_t)i * kMlpRowHalves, y);
+    mlp_apply<float>(p, y, q, e, s, h);
   }
+#pragma unroll
+  for (int c = 0; c < 6; c++) s_out[(long)i * 6 + c] = s[c];
+  ep_out[i] = e;
+  how[i] = h;
 }
 
 // ---------------------------------------------------------------------------
@@ -505,6 +514,7 @@ void stat_of(const std::vector<double> &v, int dim, int n, int c, float &mean, f
 
 int upload_view(cpi_mlp *m) {
   MlpParams h{};
+  h.weights = m->W;
   h.w1 = m->W->w1;
   h.w2p = m->W->w2p;
   h.w3p = m->W->w3p;
@@ -552,7 +562,8 @@ int ensure_data(cpi_mlp *m, size_t n) {
   m->dIdx = nullptr;
   m->data_cap = 0;
   int rc;
-  if ((rc = mdalloc(m, &m->dX, 16 * n)) || (rc = mdalloc(m, &m->dT, 8 * n)) ||
+  // dT doubles as the inference queue, whose last tile may read 15 rows past n
+  if ((rc = mdalloc(m, &m->dX, 16 * n)) || (rc = mdalloc(m, &m->dT, 8 * (n + 16))) ||
       (rc = mdalloc(m, &m->dIdx, n)))
     return rc;
   m->data_cap = n;
@@ -785,11 +796,14 @@ extern "C" int cpi_mlp_forward(cpi_mlp *m, const float *x, int32_t n, float *y) 
   (void)hipSetDevice(m->dev);
   int rc = upload_rows(m, x, nullptr, n);
   if (rc) return rc;
-  float *dY = m->dT;  // [n][8] output rows
-  hipLaunchKernelGGL(k_mlp_forward, dim3((n + 63) / 64), dim3(64), 0, m->stream, m->dP, m->dX, n, dY);
+  // dT's [rows][8] floats are the queue rows: BF16 inputs in, outputs back
+  uint16_t *rows = reinterpret_cast<uint16_t *>(m->dT);
+  hipLaunchKernelGGL(k_mlp_pack_rows, dim3((n + 255) / 256), dim3(256), 0, m->stream, m->dX, n,
+                     rows);
   MCHK(m, hipGetLastError());
+  MCHK(m, launch_mlp_rows(m->dP, rows, n, m->stream));
   std::vector<float> hy((size_t)8 * n);
-  MCHK(m, hipMemcpyAsync(hy.data(), dY, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, m->stream));
+  MCHK(m, hipMemcpyAsync(hy.data(), m->dT, sizeof(float) * hy.size(), hipMemcpyDeviceToHost, m->stream));
   MCHK(m, hipStreamSynchronize(m->stream));
   for (int i = 0; i < n; i++)
     for (int k = 0; k < kMlpOut; k++) y[(size_t)i * kMlpOut + k] = hy[(size_t)i * 8 + k];
@@ -890,7 +904,9 @@ extern "C" int cpi_mlp_return_map(cpi_mlp *m, double tol, int32_t n, const float
   (void)hipSetDevice(m->dev);
   float *d = nullptr;
   int *dh = nullptr;
-  const size_t nf = (size_t)n * 20;
+  // de, so, ep, s_out, ep_out, then the queue rows (8 floats each, one
+  // tile of padding)
+  const size_t nf = (size_t)n * 20 + ((size_t)n + 16) * 8;
   int rc;
   if ((rc = mdalloc(m, &d, nf))) return rc;
   if ((rc = mdalloc(m, &dh, (size_t)n))) {
@@ -898,6 +914,7 @@ extern "C" int cpi_mlp_return_map(cpi_mlp *m, double tol, int32_t n, const float
     return rc;
   }
   float *dde = d, *dso = d + 6L * n, *dep = d + 12L * n, *ds = d + 13L * n, *dept = d + 19L * n;
+  uint16_t *rows = reinterpret_cast<uint16_t *>(d + 20L * n);
   Params p{};
   p.c11 = m->mm.c11;
   p.c12 = m->mm.c12;
@@ -907,14 +924,21 @@ extern "C" int cpi_mlp_return_map(cpi_mlp *m, double tol, int32_t n, const float
   p.H = m->mm.H;
   p.mlp = m->dP;
   p.mlp_tol = tol;
-  p.mlp_stage_offset = 0;
-  hipError_t e = hipMemcpyAsync(dde, de, sizeof(float) * 6 * n, hipMemcpyHostToDevice, m->stream);
+  const int grid = (n + 255) / 256;
+  hipError_t e = hipMemsetAsync(rows, 0, ((size_t)n + 16) * 32, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dde, de, sizeof(float) * 6 * n, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dso, so, sizeof(float) * 6 * n, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dep, ep, sizeof(float) * n, hipMemcpyHostToDevice, m->stream);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mlp_return_points, dim3((n + 63) / 64), dim3(64), kMlpStageBytes,
-                       m->stream, p, n, (const float *)dde, (const float *)dso, (const float *)dep,
-                       ds, dept, dh);
+    hipLaunchKernelGGL(k_rm_prepare, dim3(grid), dim3(256), 0, m->stream, p, n,
+                       (const float *)dde, (const float *)dso, (const float *)dep, rows);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = launch_mlp_rows(m->dP, rows, n, m->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rm_finish, dim3(grid), dim3(256), 0, m->stream, p, n, (const float *)dde,
+                       (const float *)dso, (const float *)dep, (const uint16_t *)rows, ds, dept,
+                       dh);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(s_out, ds, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, m->stream);

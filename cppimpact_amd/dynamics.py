@@ -335,6 +335,13 @@ class Dynamics:
         L.check(L.lib().cpi_samples_enable(self.ctx, int(capacity)), self.ctx)
         self._sample_cap = int(capacity)
 
+    def subsample_samples(self, keep, seed=0):
+        """Records only the yielding points whose hash of (seed, step,
+        element, point) falls in the fraction keep (see
+        cpi_samples_subsample)."""
+        self._create()
+        L.check(L.lib().cpi_samples_subsample(self.ctx, float(keep), int(seed)), self.ctx)
+
     def samples(self):
         """(recorded samples [k][20] float32, number of yielding points seen)."""
         cap = getattr(self, "_sample_cap", 0)

@@ -10,6 +10,7 @@
 //   --ep               J2 elastoplastic force model instead of the reference's
 //   --dump PREFIX      write PREFIX_x.txt / PREFIX_v.txt (final state, %.17g)
 //   --out DIR          VTK export directory (default ../gpu_output)
+//   --export-interval N  export (and probe node 96) every N loop iterations
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -49,6 +50,7 @@ int main(int argc, char *argv[]) {
 
   bool smoke_test = false, ep = false;
   std::string filename("../input/fuselage 5086 elements.inp"), dump_prefix, out_dir;
+  int export_every = 0;
   T vz = -10, location = 0.15 - 0.00005;
   double dt = 1e-7, time_end = -1;
   for (int i = 1; i < argc; i++) {
@@ -76,6 +78,8 @@ int main(int argc, char *argv[]) {
       dump_prefix = next();
     } else if (a == "--out") {
       out_dir = next();
+    } else if (a == "--export-interval") {
+      export_every = std::atoi(next().c_str());
     }
   }
   if (time_end < 0) time_end = smoke_test ? dt * 10 : 0.5;
@@ -109,6 +113,7 @@ int main(int argc, char *argv[]) {
 #ifdef CPPIMPACT_DEBUG_MODE
   export_interval = 100;
 #endif
+  if (export_every > 0) export_interval = export_every;
 
   Wall<T, 2, Basis> w(wall_name, location, E * 10, tensile.slave_nodes, tensile.num_slave_nodes,
                       normal);

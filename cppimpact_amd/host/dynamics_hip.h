@@ -337,13 +337,18 @@ class Dynamics {
       chunk = std::min<long long>(chunk, 1 << 30);
       cpiErrchk(cpi_step(ctx, (int32_t)chunk), ctx);
       done += chunk;
-      timestep += (int)chunk;
+      // iteration k = timestep + chunk - 1 just ended: as in the reference,
+      // it exports and probes under its own index (dynamics.h:505-508), so
+      // a probe file starts afresh at iteration 0; the probes' time is the
+      // simulated time of the state they print, (k + 1) dt
+      timestep += (int)chunk - 1;
       time_ += chunk * dt;
-      if (export_interval > 0 && (timestep - 1) % export_interval == 0) {
+      if (export_interval > 0 && timestep % export_interval == 0) {
         fetch(ctx);
-        export_to_vtk(timestep - 1, vel_i, global_acc, global_mass);
+        export_to_vtk(timestep, vel_i, global_acc, global_mass);
         probe_node(96);
       }
+      timestep += 1;
     }
     fetch(ctx);
     cpi_destroy(ctx);

@@ -158,3 +158,77 @@ def collect_samples(mesh, material, wall, speeds, dt, nsteps, capacity=1 << 21,
     # the recorder fills slots in atomic-counter order; sorting the rows makes
     # the set (and so the training run) independent of that order
     return smp[np.lexsort(smp.T[::-1])]
+
+
+def workload_training_set(workloads, per_workload=3, target=10_000_000, seed=0, low=5.0,
+                          high=30.0, precision=L.PREC_F64, log=None):
+    """Return-mapping samples of the plain solver on whole workloads.
+
+    Each workload (cppimpact_amd.workloads) runs its full length with the
+    exact return at per_workload impact speeds drawn uniformly from
+    [low, high] m/s (numpy default_rng(seed)).  A first run of every speed
+    only counts the yielding points; each workload that yields then gets an
+    equal share of target, which its runs keep as a fixed hash-selected
+    fraction of their points (cpi_samples_subsample), so the set is about
+    target samples, spread over the workloads and over the whole of each
+    run, and the same in every process.  Returns (samples [k][20] sorted,
+    info) with info listing the speeds, yielding points and kept samples
+    per run."""
+    rng = np.random.default_rng(seed)
+    plan = [(w, [float(v) for v in rng.uniform(low, high, per_workload)]) for w in workloads]
+    runs = []
+
+    def one(dyn, w, v, cap, keep):
+        dyn.vel[:] = np.asarray(w.direction, dtype=np.float64) * v
+        dyn.reset_samples()
+        dyn.subsample_samples(keep, seed=seed * 1000003 + len(runs))
+        dyn.init(0.0)
+        dyn.step(w.steps)
+        dyn.synchronize()
+        return dyn.samples()
+
+    # pass 1: yielding points per run (nothing recorded, only counted)
+    counts = {}
+    for w, speeds in plan:
+        dyn = w.dynamics(precision=precision)
+        dyn.enable_samples(1)
+        for v in speeds:
+            _, seen = one(dyn, w, v, 1, 1.0)
+            counts[(w.name, v)] = seen
+        dyn.close()
+    yielding = [w.name for w, speeds in plan if sum(counts[(w.name, v)] for v in speeds) > 0]
+    if not yielding:
+        return np.zeros((0, L.SAMPLE_FLOATS), np.float32), {"runs": []}
+    share = target / len(yielding)
+    out = []
+    for w, speeds in plan:
+        total = sum(counts[(w.name, v)] for v in speeds)
+        if total == 0:
+            for v in speeds:
+                runs.append(dict(workload=w.name, speed=round(v, 3), yielding=0, kept=0))
+            continue
+        keep = min(1.0, share / total)
+        cap = int(max(counts[(w.name, v)] for v in speeds) * keep * 1.05) + 4096
+        dyn = w.dynamics(precision=precision)
+        dyn.enable_samples(cap)
+        for v in speeds:
+            s, seen = one(dyn, w, v, cap, keep)
+            if seen > len(s):
+                # the recorder overflowed: again with room for every kept point
+                dyn.enable_samples(seen)
+                s, seen2 = one(dyn, w, v, seen, keep)
+                if seen2 != seen or len(s) != seen:
+                    raise RuntimeError(f"workload_training_set: {seen} then {seen2} kept points "
+                                       f"for {w.name} at {v} m/s")
+                dyn.enable_samples(cap)
+            runs.append(dict(workload=w.name, speed=round(v, 3), yielding=counts[(w.name, v)],
+                             kept=len(s), keep_fraction=keep))
+            if log:
+                log(f"training set: {w.name} at {v:.2f} m/s: {counts[(w.name, v)]} yielding "
+                    f"points, {len(s)} kept")
+            out.append(s)
+        dyn.close()
+    smp = np.concatenate(out) if out else np.zeros((0, L.SAMPLE_FLOATS), np.float32)
+    smp = smp[np.lexsort(smp.T[::-1])]
+    return smp, {"runs": runs, "samples": int(len(smp)), "target": int(target),
+                 "seed": int(seed), "speed_range": [low, high]}

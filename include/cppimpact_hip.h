@@ -40,7 +40,8 @@ typedef enum cpi_status {
   CPI_ERR_HIP = -2,         /* HIP runtime failure (gpuErrchk analogue) */
   CPI_ERR_ALLOC = -3,       /* host or device allocation failed */
   CPI_ERR_MESH = -4,        /* degenerate/inverted element, bad connectivity */
-  CPI_ERR_NUMERIC = -5,     /* NaN/Inf detected in the state */
+  CPI_ERR_NUMERIC = -5,     /* NaN/Inf in the state, an inverted element, or
+                               a dt above the stable limit (J2 physics) */
   CPI_ERR_IO = -6,          /* file could not be opened or parsed */
   CPI_ERR_STATE = -7,       /* call made out of order (e.g. step before init) */
   CPI_ERR_NO_DEVICE = -8    /* no usable gfx950 device */
@@ -75,8 +76,9 @@ typedef enum cpi_status {
 
 /* stress update used by CPI_PHYSICS_EP */
 #define CPI_RETURN_EXACT 0   /* closed-form radial return */
-#define CPI_RETURN_MLP 1     /* MLP surrogate, exact return where its yield
-                                residual exceeds mlp_tolerance * Y0 */
+#define CPI_RETURN_MLP 1     /* MLP surrogate; the exact return (FP64) where
+                                its state misses a J2 condition by more than
+                                mlp_tolerance * Y0 (see cpi_set_surrogate) */
 
 /* energy row layout (cpi_get_energy) */
 #define CPI_E_KINETIC 0   /* 1/2 sum m |v_n|^2 */
@@ -142,6 +144,11 @@ typedef struct cpi_options {
                                velocity, zero acceleration, no contact);
                                0 = none, as the reference (which collects
                                the set and never applies it) */
+  int32_t debug_fill_nan;   /* 1: fill every device buffer of the context with
+                               all-ones bytes (NaN as a float or double, -1 as
+                               an integer) when it is allocated, so a read of
+                               memory the context never wrote shows as a
+                               non-finite state (diagnostics; default 0) */
   double gravity[3];        /* default (0, 0, -9.81): dynamics_kernels.h:165-167 */
   double mlp_tolerance;     /* default 1e-3 */
 } cpi_options;
@@ -204,7 +211,11 @@ int cpi_init(cpi_ctx *ctx, double dt);
 int cpi_step(cpi_ctx *ctx, int32_t nsteps);
 
 /* Waits for queued work; reports CPI_ERR_NUMERIC if a NaN/Inf or an inverted
-   element was seen. */
+   element was seen, or, with CPI_PHYSICS_EP, if the fixed dt exceeded the
+   stable limit of a configuration stepped from (the element CFL limit,
+   reduced on the device on the last step of every cpi_step call, or the
+   wall spring's 2 sqrt(m_min / k)).  The reference steps with a fixed dt
+   and no such check (test.cu:83); dt is never changed here either. */
 int cpi_synchronize(cpi_ctx *ctx);
 
 /* Host copies of Dynamics::global_xloc, vel (v_{n+1/2}), global_acc (a_n),
@@ -246,9 +257,9 @@ int cpi_time_steps(cpi_ctx *ctx, int32_t nsteps, double *elapsed_ms);
 /* Runs nsteps steps launching each kernel separately with HIP events
    around every launch on the context's stream; fills per-kernel totals:
    the element pass ("elem_block" / "elem_direct") and the node pass
-   ("node_bnd" / "node_all"), then, in the surrogate mode before the switch
-   to the surrogate kernel alone, its two launches one by one ("elem_pass",
-   "mlp_walk").  *count = entries written (at most cap). */
+   ("node_bnd" / "node_all"), then, in the surrogate mode, the element
+   pass's three launches one by one on every other step ("elem_pass",
+   "mlp_queue", "mlp_walk").  *count = entries written (at most cap). */
 int cpi_profile_kernels(cpi_ctx *ctx, int32_t nsteps, cpi_kernel_time *out,
                         int32_t cap, int32_t *count);
 
@@ -270,8 +281,13 @@ int cpi_bandwidth_probe(int32_t device, int64_t bytes, int32_t reps, double *gbs
  * K_e du of dynamics_kernels.h:101-114).  A 13 -> 64 -> 64 -> 7 SiLU network
  * evaluated with BF16 MFMA (FP32 accumulation) replaces the radial return
  * at yielding points of CPI_PHYSICS_EP + CPI_RETURN_MLP contexts (tet4,
- * FP32, SCATTER_BLOCK); a point whose surrogate state misses the yield
- * surface by more than mlp_tolerance * Y0 takes the exact return.
+ * FP32, SCATTER_BLOCK).  Its state is accepted only where it keeps the
+ * pressure, does not turn the trial deviator, lowers the deviator's norm by
+ * sqrt(6) G dep (consistency) and lies on the yield surface, each within
+ * mlp_tolerance * Y0; elsewhere the point takes the exact radial return in
+ * FP64.  A step queues the yielding points of the blocks whose trial stress
+ * yields, evaluates the network on the whole queue in one launch and
+ * finishes those blocks with its outputs.
  *
  * Inputs (standardized): strain increment [6], stress_n [6], eq. plastic
  * strain_n.  Outputs (standardized): (sigma_trial - sigma_{n+1}) / q [6] and
@@ -340,7 +356,8 @@ int cpi_mlp_train(cpi_mlp *m, const float *samples, int32_t n,
                   const cpi_mlp_train_opts *opt, float *loss_history);
 
 /* The element pass's surrogate return map on n points (FP32): trial
-   stress sigma_n + D de, then surrogate or exact return; how[i] is
+   stress sigma_n + D de, then the surrogate's state where it passes the
+   acceptance conditions above, else the exact FP64 return; how[i] is
    CPI_MLP_ELASTIC / ACCEPTED / FALLBACK. */
 int cpi_mlp_return_map(cpi_mlp *m, double tolerance, int32_t n,
                        const float *de, const float *sigma_n,
@@ -360,6 +377,13 @@ int cpi_reset_surrogate_stats(cpi_ctx *ctx);
    of capacity samples (0 disables); points past the capacity are counted
    and dropped. */
 int cpi_samples_enable(cpi_ctx *ctx, int32_t capacity);
+/* Keeps only a fixed subset of the yielding points: point q of element e in
+   step n is recorded when a 32-bit hash of (seed, n, e, q) falls below
+   keep * 2^32 (keep = 1, the default, records every point).  The subset
+   depends on the run alone, not on the order of the atomic slot counter,
+   so a training set drawn from long runs is reproducible; the counter then
+   counts the kept points. */
+int cpi_samples_subsample(cpi_ctx *ctx, double keep, uint64_t seed);
 /* copies min(recorded, capacity, cap) samples; *recorded = points seen */
 int cpi_samples_get(cpi_ctx *ctx, float *out, int32_t cap, int32_t *recorded);
 int cpi_samples_reset(cpi_ctx *ctx);

@@ -15,6 +15,10 @@ mathematics instead of reference vectors:
     order rather than by BF16 rounding;
   * the J2 radial return of Simo & Hughes (1998) Box 3.2 with linear
     isotropic hardening, pinned by its closed-form consistency conditions;
+  * the surrogate's acceptance conditions (the four properties that fix a
+    J2 return: pressure kept, no turn of the deviator, consistency, yield),
+    pinned by the exact return meeting them to rounding and by planted
+    mistakes each failing its own condition;
   * the sample -> (x, t) map of the trainer (overshoot-scaled targets).
 
 Parameter order: W1[64][13], b1[64], W2[64][64], b2[64], W3[7][64], b3[7].
@@ -197,3 +201,47 @@ def surrogate_state(y_std, s_trial, ep, out_mean, out_std, Y0, H):
 
 def yield_residual(s, ep, Y0, H):
     return dev_norm(s) - K23 * (Y0 + H * np.asarray(ep, dtype=np.float64))
+
+
+def voigt_dot(a, b):
+    """a : b for Voigt vectors with engineering shear stored as stress
+    (shear terms counted twice)."""
+    return (np.sum(a[..., :3] * b[..., :3], axis=-1)
+            + 2.0 * np.sum(a[..., 3:] * b[..., 3:], axis=-1))
+
+
+def deviator(s):
+    d = np.array(s, dtype=np.float64, copy=True)
+    p = (d[..., 0] + d[..., 1] + d[..., 2]) / 3.0
+    d[..., :3] -= p[..., None]
+    return d
+
+
+def acceptance_residuals(s_trial, s_new, ep, ep_new, E, nu, Y0, H):
+    """The four J2 conditions a return map (s_trial, ep) -> (s_new, ep_new)
+    meets exactly (Simo & Hughes 1998, Box 3.2: the return is radial,
+    deviatoric, consistent and ends on the yield surface), as residuals in
+    stress units: (pressure change, turn of the deviator, consistency,
+    yield).  The surrogate's state is accepted when all four are within
+    tol * Y0."""
+    _, _, _, G = elastic_moduli(E, nu)
+    s_trial = np.asarray(s_trial, dtype=np.float64)
+    s_new = np.asarray(s_new, dtype=np.float64)
+    dep = np.asarray(ep_new, dtype=np.float64) - np.asarray(ep, dtype=np.float64)
+    c = s_trial - s_new
+    pc = (c[..., 0] + c[..., 1] + c[..., 2]) / 3.0
+    dt, dc = deviator(s_trial), deviator(c)
+    nt = np.sqrt(voigt_dot(dt, dt))
+    proj = voigt_dot(dc, dt) / nt
+    turn = np.sqrt(np.maximum(voigt_dot(dc, dc) - proj * proj, 0.0))
+    nn = np.sqrt(voigt_dot(dt - dc, dt - dc))
+    consistency = (nt - nn) - np.sqrt(6.0) * G * dep
+    return (np.abs(pc), turn, np.abs(consistency),
+            np.abs(yield_residual(s_new, ep_new, Y0, H)))
+
+
+def surrogate_accepts(s_trial, s_new, ep, ep_new, E, nu, Y0, H, tol):
+    """Whether the surrogate's proposal passes its acceptance checks."""
+    r = acceptance_residuals(s_trial, s_new, ep, ep_new, E, nu, Y0, H)
+    lim = tol * Y0
+    return (r[0] <= lim) & (r[1] <= lim) & (r[2] <= lim) & (r[3] <= lim)

@@ -6,9 +6,13 @@ c1: 20^3-hex cube, 48,000 tet10 (4-point rule), -20 m/s: the first steps
     against the oracle on the full mesh, then 2,000 GPU steps checked
     through the energy balance and FP32 against FP64.
 c2: 64^3-hex cube, 1,572,864 tet4 jittered by 0.1 of the spacing (seed 0),
-    -10 m/s, the throughput workload: the first steps against the oracle on
-    the full mesh, then 241 steps of FP64, FP32 and FP32 with the surrogate
-    (the impact stays elastic, so the network is never evaluated).
+    -10 m/s, the throughput workload: its 500 steps against the oracle on
+    the full mesh in FP64 and FP32, then 241 steps of FP64, FP32 and FP32
+    with the surrogate (the impact stays elastic, so the network is never
+    evaluated).
+c4: the surrogate on the c2 mesh where the cube yields (30 m/s), trained on
+    return-mapping samples of c1-c3 runs, against the FP64 and FP32 exact
+    runs over c2's 500 steps.
 c3: thin-walled cylinder R 1 m, L 4 m, t 0.02 m, 360 x 185 x 3 cells x 6
     tet10 (1,198,800 elements), oblique 7 m/s impact 30 degrees from the
     wall normal, 1,000 steps, FP32 against FP64.
@@ -118,7 +122,8 @@ def test_c1_two_thousand_steps_energy_and_fp32():
           + ", ".join(f"{n}: {a:.1e}" for n, a in ctrl.items()))
     assert np.isfinite(e).all()
     assert ep64.max() > 0  # 20 m/s yields
-    assert abs(e[L.E_BALANCE]) < 2e-2 * ke0
+    # measured 8.8e-6 KE0 while 33% of KE0 is dissipated plastically
+    assert abs(e[L.E_BALANCE]) < 1e-4 * ke0
     # FP32 element arithmetic: the gap starts at rounding level and grows
     # with the plastic flow (the yield overshoot is a small difference of
     # large stresses); measured 2.2e-2 after 2000 steps
@@ -136,17 +141,37 @@ def c2_mesh():
     return x, conn, 10 * E * 0.25 / 64
 
 
-def test_c2_first_steps_against_oracle_on_full_mesh():
+def test_c2_whole_run_against_oracle_on_full_mesh():
+    """c2 for its stated 500 steps: FP64 against the oracle (its threaded
+    solve sums every node in element order, bit-identical to the serial
+    one) to 1e-9 in displacement and velocity and 1e-8 of KE0 in the
+    energies, and the FP32 element kernels against the same oracle to
+    1e-3."""
+    import os
+
     x, conn, k = c2_mesh()
     v0 = (0.0, 0.0, -10.0)
     d = dynamics(x, conn, v0, k)
-    g = run(d, 0.0, 3)
+    g = run(d, 0.0, 500)
+    d32 = dynamics(x, conn, v0, k, precision=L.PREC_F32)
+    g32 = run(d32, g["dt"], 500)
+    threads = max(1, min(16, os.cpu_count() or 1))
     o = O.Solver(x, conn, v0, E=E, nu=NU, rho=RHO, sigma_y=SY, H=H, physics=O.EP,
-                 wall=dict(location=0.0, stiffness=k), dt=g["dt"]).solve(3)
+                 wall=dict(location=0.0, stiffness=k), dt=g["dt"]).solve(500, threads=threads)
+    ke0 = 0.5 * o["m"].sum() * 10.0 ** 2
+    derr, verr = rel(g["x"] - x, o["x"] - x), rel(g["v"], o["v"])
+    d32err = rel(g32["x"] - x, o["x"] - x)
+    print(f"c2 500 steps: fp64 disp {derr:.2e} v {verr:.2e}; fp32 disp {d32err:.2e}; "
+          f"balance {g['energy'][L.E_BALANCE] / ke0:.2e} KE0")
     assert rel(g["m"], o["m"]) < 1e-12
-    assert rel(g["x"] - x, o["x"] - x) < 1e-9
-    assert rel(g["v"], o["v"]) < 1e-9
+    assert derr < 1e-9 and verr < 1e-9
+    eo = o["energy"][-1]
+    for k_ in (O.E_KINETIC, O.E_INTERNAL, O.E_PENALTY, O.E_GRAVITY):
+        assert abs(g["energy"][k_] - eo[k_]) <= 1e-8 * ke0, k_
+    assert abs(g["energy"][L.E_CONTACTF] - eo[O.E_CONTACTF]) <= 1e-8 * abs(eo[O.E_CONTACTF])
+    assert d32err < 1e-3
     d.close()
+    d32.close()
 
 
 def test_c2_fp32_and_surrogate_modes_against_fp64():
@@ -185,7 +210,8 @@ def test_c2_fp32_and_surrogate_modes_against_fp64():
     # the surrogate mode's elastic pass rounds as the FP32 kernel's elastic
     # path does: the same positions bit for bit
     assert np.array_equal(xml, x32)
-    assert np.isfinite(e).all() and abs(e[L.E_BALANCE]) < 1e-2 * ke0
+    # measured 5.8e-5 KE0
+    assert np.isfinite(e).all() and abs(e[L.E_BALANCE]) < 5e-4 * ke0
     for d in (d64, d32, dml):
         d.close()
 
@@ -235,3 +261,60 @@ def test_c3_cylinder_oblique_impact_fp32_against_fp64():
     assert derr < 1e-3
     d64.close()
     d32.close()
+
+
+def test_c4_surrogate_at_work_on_c2_mesh():
+    """c4: the surrogate on the c2 mesh at 30 m/s, where the cube yields from
+    the first steps, over c2's 500 steps.  The network is trained as c4
+    states, on the plain solver's return-mapping samples of c1, c2 and c3
+    runs at speeds drawn uniformly from [5, 30] m/s (two per workload and
+    2M samples here, against bench.py's three and 10M, to keep the test
+    short).  The network replaces the return map at nearly every yielding
+    point; displacement and plastic strain stay within the stated bounds
+    of the plain FP64 run (measured with the 10M-sample network: L2 6.5e-5
+    and max 7.8e-3 in displacement, L2 1.4e-3 in plastic strain; the FP32
+    exact return alone is at L2 2.1e-6, max 6.6e-4)."""
+    from cppimpact_amd import workloads as W
+    from cppimpact_amd.surrogate import Surrogate, workload_training_set
+
+    smp, info = workload_training_set([W.c1(), W.c2(), W.c3()], per_workload=2,
+                                      target=2_000_000, seed=0)
+    kept = {}
+    for r in info["runs"]:
+        kept[r["workload"]] = kept.get(r["workload"], 0) + r["kept"]
+    assert len(smp) > 1_000_000 and min(kept.values()) > 0, kept
+    net = Surrogate(material(), seed=0)
+    hist = net.train(smp, epochs=10, batch=1024)
+    assert hist[-1] < 0.05 * hist[0]
+    x, conn, k = c2_mesh()
+    v0 = (0.0, 0.0, -30.0)
+    d64 = dynamics(x, conn, v0, k)
+    d32 = dynamics(x, conn, v0, k, precision=L.PREC_F32)
+    dml = dynamics(x, conn, v0, k, precision=L.PREC_F32, stress_update=L.RETURN_MLP)
+    dml.set_surrogate(net)
+    d64.init(0.0)
+    for d in (d32, dml):
+        d.init(d64.info().dt)
+    for d in (d64, d32, dml):
+        d.step(500)
+    x64, x32, xml = d64.fetch()[0], d32.fetch()[0], dml.fetch()[0]
+    _, ep64 = d64.element_state()
+    _, epml = dml.element_state()
+    acc, fb = dml.surrogate_stats()
+
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    dl2, dmx = l2(xml - x, x64 - x), rel(xml - x, x64 - x)
+    el2 = l2(epml, ep64)
+    print(f"c4 30 m/s 500 steps: {acc} accepted, {fb} fell back; surrogate vs fp64 disp L2 "
+          f"{dl2:.2e} max {dmx:.2e}, ep L2 {el2:.2e}; fp32 vs fp64 disp L2 "
+          f"{l2(x32 - x, x64 - x):.2e} max {rel(x32 - x, x64 - x):.2e}; ep max {ep64.max():.3e}")
+    assert ep64.max() > 1e-3
+    assert acc > 10_000_000 and acc >= 0.95 * (acc + fb)
+    assert dl2 < 1e-3 and dmx < 3e-2 and el2 < 2e-2
+    assert l2(x32 - x, x64 - x) < 1e-4
+    assert np.isfinite(dml.energy()[0]).all()
+    for d in (d64, d32, dml):
+        d.close()
+    net.close()

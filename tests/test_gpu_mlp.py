@@ -1,7 +1,8 @@
 """GPU tests of the MLP surrogate: the BF16-MFMA training gradient and the
 inference kernel against the numpy oracle, the trainer's target map, the
-return map's accept / fall-back logic, and the element pass with the
-surrogate against the exact return (ablation)."""
+return map's acceptance checks (against the oracle's, and on planted
+mistakes) and FP64 fall-back, and the element pass with the surrogate
+against the exact return (ablation)."""
 import numpy as np
 import pytest
 
@@ -185,18 +186,21 @@ def test_surrogate_step_against_exact_return():
     f = out["forced"]
     assert f["accepted"] == 0 and f["fallback"] > 0
     assert f["disp"] < 2e-4 and f["stress"] < 2e-3 and f["ep"] < 2e-3
-    # the residual check rejects much of the untrained network's output (it
-    # accepts points whose overshoot q is itself below the tolerance, where
-    # any bounded correction is within it) ...
+    # the acceptance checks send most of the untrained network's points back
+    # to the exact return (it passes where the overshoot q is itself below
+    # the tolerance, so any bounded correction is within it; measured 19%
+    # accepted) ...
     u = out["untrained"]
-    assert u["fallback"] >= 0.3 * (u["accepted"] + u["fallback"])
+    assert u["fallback"] >= 0.7 * (u["accepted"] + u["fallback"])
     # ... the trained one replaces the return map at nearly every yielding
-    # point, stays near the exact run, and is several times closer to it
+    # point (measured: all of them), five times as many points as the
+    # untrained one, and stays closer to the exact run
     t = out["trained"]
     assert t["accepted"] >= 0.9 * (t["accepted"] + t["fallback"]) > 0
+    assert t["accepted"] >= 4 * u["accepted"]
     assert t["disp"] < 1e-2 and t["ep"] < 2e-2 and t["stress"] < 6e-2 and t["energy_ok"]
     for k in ("disp", "stress", "ep"):
-        assert 3 * t[k] < u[k]
+        assert 2 * t[k] < u[k]
     ref.close()
     net.close()
     raw.close()
@@ -229,14 +233,12 @@ def test_surrogate_step_is_bitwise_reproducible():
     net.close()
 
 
-def test_surrogate_elastic_pass_handover_is_bitwise_stable():
-    """Surrogate steps run an elastic pass until the first yield, hand
-    yielding blocks to the surrogate kernel, and switch to the surrogate
-    kernel alone once the host sees the first yield.  When the host sees it
-    must not matter: one 80-step call (the switch comes late, deferred
-    blocks go through the list walk meanwhile), 80 one-step calls (the
-    switch comes at once) and an eager run agree bit for bit on a mesh of
-    many blocks, counters included."""
+def test_surrogate_step_is_independent_of_graph_chunking():
+    """A surrogate step queues the yielding points of the blocks it defers
+    in atomic order and finishes the listed blocks in whatever workgroup
+    draws them; none of that may show in the results.  One 80-step call
+    (graphs of 32 and 4 steps), 80 one-step calls and an eager run agree bit
+    for bit on a mesh of many blocks, counters included."""
     x, conn = cube(10, gap=2e-6, jitter=0.1, seed=5)
     smp = collect_samples(ci.Mesh.from_arrays(x, conn), material(),
                           ci.Wall("Wall", 0.0, wall_k(0.25, 6), None, 0, 1), [30.0], 0.0, 40)
@@ -281,3 +283,97 @@ def test_collected_set_and_training_are_reproducible():
         net.train(a, epochs=3, batch=256)
         params.append(net.params.copy())
     assert np.array_equal(params[0], params[1])
+
+
+def _zero_network(out_mean):
+    """A network whose every output is exactly out_mean: all weights and
+    biases zero (SiLU(0) = 0), unit output scale."""
+    s = Surrogate(material(), seed=0)
+    s.params = np.zeros(L.MLP_NPARAMS, np.float32)
+    s.set_scaling(np.zeros(L.MLP_INPUTS), np.ones(L.MLP_INPUTS), out_mean, np.ones(L.MLP_OUTPUTS))
+    return s
+
+
+def test_acceptance_checks_on_planted_mistakes():
+    """Points whose trial deviators share one direction n and one overshoot
+    q (pressures differ): the exact return's scaled correction is the
+    constant (2G/(2G + 2H/3) n, sqrt(2/3) Y0/(2G + 2H/3)), so a zero network
+    with that output mean proposes the exact return, and every point is
+    accepted.  Offsets of three tolerances planted in the output mean --
+    along the pressure, across the trial direction, in the plastic strain,
+    along the direction -- each fail, and the point falls back to the exact
+    FP64 return."""
+    rng = np.random.default_rng(60)
+    n = 512
+    G = E / (2 * (1 + NU))
+    nhat = rng.normal(size=6)
+    nhat[:3] -= nhat[:3].mean()
+    nhat /= np.sqrt(M.voigt_dot(nhat, nhat))
+    ep0 = 0.01
+    q = 0.05 * SY
+    r = M.K23 * (SY + H * ep0) + q
+    so = rng.uniform(-0.5, 0.5, n)[:, None] * SY * np.array([1, 1, 1, 0, 0, 0]) + r * nhat
+    de = np.zeros((n, 6))
+    ep = np.full(n, ep0)
+    kappa = 2 * G / (2 * G + 2 * H / 3)
+    exact = np.concatenate([kappa * nhat, [SY * M.K23 / (2 * G + 2 * H / 3)]])
+    tol = 1e-3
+    ex_s, ex_e, _ = M.radial_return(so, ep, E, NU, SY, H)
+    net = _zero_network(exact)
+    s1, e1, how = net.return_map(de, so, ep, tolerance=tol)
+    assert np.all(how == L.MLP_ACCEPTED)
+    assert rel(s1, ex_s, SY) < 2e-6 and np.abs(e1 - ex_e).max() < 1e-6 * ep0
+    net.close()
+    off = 3 * tol * SY / q
+    u = rng.normal(size=6)
+    u[:3] -= u[:3].mean()
+    u -= M.voigt_dot(u, nhat) * nhat
+    u /= np.sqrt(M.voigt_dot(u, u))
+    planted = {
+        "pressure": np.concatenate([[off, off, off, 0, 0, 0], [0.0]]),
+        "turn": np.concatenate([off * u, [0.0]]),
+        # dep off by 3 tol Y0 / (sqrt(6) G): the consistency condition
+        "plastic strain": np.concatenate([np.zeros(6), [off * SY / (np.sqrt(6.0) * G)]]),
+        "radial": np.concatenate([off * nhat, [0.0]]),
+    }
+    for name, d in planted.items():
+        net = _zero_network(exact + d)
+        s2, e2, how2 = net.return_map(de, so, ep, tolerance=tol)
+        assert np.all(how2 == L.MLP_FALLBACK), name
+        # the fall-back is the exact return in FP64, rounded to FP32
+        assert rel(s2, ex_s, SY) < 2e-7 and np.abs(e2 - ex_e).max() < 2e-7 * ep0, name
+        net.close()
+
+
+def test_acceptance_decisions_match_oracle():
+    """A trained network's proposals on random plastic states: the GPU
+    accepts exactly where the oracle's four conditions hold (points within
+    2% of the tolerance on any condition are left out: FP32 against FP64),
+    and accepted states are the network's proposals."""
+    rng = np.random.default_rng(70)
+    smp = exact_samples(rng, 30000)
+    net = Surrogate(material(), seed=9)
+    net.train(smp, epochs=10, batch=512)
+    de, so, ep = plastic_states(np.random.default_rng(71), 4000)
+    de, so, ep = (a.astype(np.float32) for a in (de, so, ep))
+    tol = 1e-3
+    s1, e1, how = net.return_map(de, so, ep, tolerance=tol)
+    tr = M.trial(de.astype(np.float64), so.astype(np.float64), E, NU)
+    ep64 = ep.astype(np.float64)
+    q = M.overshoot(tr, ep64, SY, H)
+    pl = q > 1e-4 * SY
+    xin = np.concatenate([de, so, ep[:, None]], axis=1).astype(np.float64)
+    im, istd, om, ostd = (a.astype(np.float64) for a in net.scaling())
+    y = net.forward(((xin - im) / istd).astype(np.float32))
+    ps, pe = M.surrogate_state(y[pl], tr[pl], ep64[pl], om, ostd, SY, H)
+    res = M.acceptance_residuals(tr[pl], ps, ep64[pl], pe, E, NU, SY, H)
+    lim = tol * SY
+    clear = np.all([np.abs(r - lim) > 0.02 * lim for r in res], axis=0)
+    want = M.surrogate_accepts(tr[pl], ps, ep64[pl], pe, E, NU, SY, H, tol)
+    got = how[pl] == L.MLP_ACCEPTED
+    assert clear.sum() > 0.9 * pl.sum()
+    assert want[clear].sum() > 100 and (~want[clear]).sum() > 20
+    assert np.array_equal(got[clear], want[clear])
+    acc = got & clear
+    assert rel(s1[pl][acc], ps[acc], SY) < 1e-5
+    net.close()

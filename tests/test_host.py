@@ -235,3 +235,20 @@ def test_cpp_driver_reports_missing_mesh():
     p = subprocess.run([exe, "--mesh", "/nonexistent/mesh.inp"], capture_output=True, text=True,
                        timeout=60)
     assert p.returncode == 1 and "Error loading mesh" in p.stderr
+
+
+def test_create_refuses_an_inverted_element():
+    """One element with two corners swapped has a negative volume: the
+    context is refused with CPI_ERR_MESH naming it (the reference only
+    prints a non-positive detJ when it meets one, analysis.h:235-239).  The
+    mesh is validated before any device is touched."""
+    import cppimpact_amd as ci
+    from cppimpact_amd import _lib as L
+    x, conn = ci.box_mesh(2, 2, 2, 0.1, 0.1, 0.1)
+    conn = conn.copy()
+    conn[5, [1, 2]] = conn[5, [2, 1]]
+    d = ci.Dynamics(ci.Mesh.from_arrays(x, conn), ci.Elastoplastic(68.9e9, 2700.0, 0.33, 1.0,
+                                                                    700e6, 276e6))
+    with pytest.raises(L.CpiError) as ex:
+        d.init(0.0)
+    assert ex.value.code == -4 and "element 5 is inverted" in str(ex.value)

@@ -138,3 +138,71 @@ def test_fit_scaling_floor():
     t = np.zeros((10, M.N_OUT))
     xm, xs, tm, ts = M.fit_scaling(x, t)
     assert np.all(xs[1:] == 1e-3) and xs[0] == 1.0 and np.all(ts == 1.0)
+
+
+def _unit_deviatoric(rng, n):
+    d = rng.normal(size=(n, 6))
+    d[:, :3] -= d[:, :3].mean(axis=1, keepdims=True)
+    return d / np.sqrt(M.voigt_dot(d, d))[:, None]
+
+
+def test_acceptance_conditions_pass_the_exact_return_and_fail_planted_mistakes():
+    """The exact J2 return meets the four acceptance conditions to rounding,
+    and each planted mistake of the size of the tolerance breaks its own
+    condition: a pressure shift (pressure), a turn of the returned deviator
+    at fixed norm (turn), a plastic-strain increment off by tol / (sqrt(6)
+    G) (consistency), a returned deviator scaled along itself off the yield
+    surface with the matching plastic strain (yield).  The turn is measured
+    against a brute-force minimum over the trial direction's line."""
+    rng = np.random.default_rng(7)
+    de, so, ep = plastic_states(rng, 600)
+    tr = M.trial(de, so, E, NU)
+    q = M.overshoot(tr, ep, SY, H)
+    pl = q > 1e-3 * SY
+    tr, ep, q = tr[pl], ep[pl], q[pl]
+    s1, ep1, dg = M.radial_return(tr, ep, E, NU, SY, H)
+    tol = 1e-3
+    lim = tol * SY
+    res = M.acceptance_residuals(tr, s1, ep, ep1, E, NU, SY, H)
+    for r in res:
+        assert r.max() < 1e-5 * lim
+    assert np.all(M.surrogate_accepts(tr, s1, ep, ep1, E, NU, SY, H, tol))
+    mu = E / (2 * (1 + NU))
+    k = 3.0 * lim
+    # pressure
+    sp = s1.copy()
+    sp[:, :3] += k
+    r = M.acceptance_residuals(tr, sp, ep, ep1, E, NU, SY, H)
+    assert np.all(r[0] > 2 * lim) and np.all(r[1] < 1e-5 * lim)
+    # turn: rotate the returned deviator towards a direction orthogonal to the
+    # trial deviator, keeping its norm
+    dt = M.deviator(tr)
+    nhat = dt / np.sqrt(M.voigt_dot(dt, dt))[:, None]
+    u = _unit_deviatoric(rng, len(tr))
+    u -= M.voigt_dot(u, nhat)[:, None] * nhat
+    u /= np.sqrt(M.voigt_dot(u, u))[:, None]
+    d1 = M.deviator(s1)
+    n1 = np.sqrt(M.voigt_dot(d1, d1))
+    ang = k / n1
+    st = s1.copy()
+    st += (np.cos(ang)[:, None] - 1.0) * d1 + (np.sin(ang) * n1)[:, None] * u
+    r = M.acceptance_residuals(tr, st, ep, ep1, E, NU, SY, H)
+    assert np.all(r[1] > 2 * lim)
+    # brute force: the distance from the turned deviator to the trial line
+    lam = np.linspace(0.9, 1.1, 20001)
+    dtt = M.deviator(st)
+    for i in range(5):
+        dist = np.sqrt(M.voigt_dot(dtt[i] - lam[:, None] * nhat[i] * n1[i], dtt[i] - lam[:, None] * nhat[i] * n1[i]))
+        assert abs(dist.min() - r[1][i]) < 1e-3 * r[1][i]
+    assert np.all(r[3] < 1e-5 * lim)  # norm kept: still on the surface
+    # consistency: dep off at an unchanged stress
+    epc = ep1 + k / (np.sqrt(6.0) * mu)
+    r = M.acceptance_residuals(tr, s1, ep, epc, E, NU, SY, H)
+    assert np.all(r[2] > 2 * lim)
+    # yield: deviator scaled along itself by k, plastic strain consistent with
+    # the new norm, so only the yield residual fails
+    sy_ = s1 - (k / n1)[:, None] * d1
+    epy = ep + (np.sqrt(M.voigt_dot(dt, dt)) - (n1 - k)) / (np.sqrt(6.0) * mu)
+    r = M.acceptance_residuals(tr, sy_, ep, epy, E, NU, SY, H)
+    assert np.all(r[3] > 2 * lim) and np.all(r[2] < 1e-5 * lim) and np.all(r[1] < 1e-5 * lim)
+    assert not np.any(M.surrogate_accepts(tr, sy_, ep, epy, E, NU, SY, H, tol))
