@@ -110,6 +110,7 @@ SIGNATURES = {
     "cpi_last_error": (C.c_char_p, [C.c_void_p]),
     "cpi_set_state": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cpi_stable_dt": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "cpi_stable_dt_noalloc": (C.c_int, [C.c_void_p, _dp, _dp]),
     "cpi_init": (C.c_int, [C.c_void_p, C.c_double]),
     "cpi_step": (C.c_int, [C.c_void_p, C.c_int32]),
     "cpi_synchronize": (C.c_int, [C.c_void_p]),

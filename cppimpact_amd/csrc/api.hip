@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -717,6 +718,90 @@ extern "C" int cpi_stable_dt(cpi_ctx *c, double *dte, double *dtc) {
     if (rc) return rc;
   }
   return stable_dt(c, dte, dtc);
+}
+
+// ---------------------------------------------------------------------------
+// cpi_stable_dt without allocation: the scratch of its reductions is a
+// file-scope device array, one copy per GPU, held until the process ends.
+// It takes the per-workgroup element minima of k_elem_dt (one per 256
+// elements) and, in its last kNoallocOut words, the two results of
+// k_dt_finish.  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr int kNoallocPartials = 65536;  // 16,777,216 elements at most
+constexpr int kNoallocOut = 2;
+static __device__ __align__(256) double scratch[kNoallocPartials + kNoallocOut];
+
+// one workgroup: the minimum of the element partials and the smallest
+// positive nodal mass (the wall spring's limit needs it)
+__global__ void __launch_bounds__(256) k_dt_finish(const double *part, int nb, const double *m,
+                                                   int nn, double *out) {
+  __shared__ double r0[256], r1[256];
+  double a = 1e300, b = 1e300;
+  for (int i = threadIdx.x; i < nb; i += 256) a = fmin(a, part[i]);
+  for (int i = threadIdx.x; i < nn; i += 256)
+    if (m[i] > 0.0) b = fmin(b, m[i]);
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      r0[threadIdx.x] = fmin(r0[threadIdx.x], r0[threadIdx.x + h]);
+      r1[threadIdx.x] = fmin(r1[threadIdx.x], r1[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = r0[0];
+    out[1] = r1[0];
+  }
+}
+
+namespace {
+
+constexpr int kNoallocDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kNoallocDevices] = {};
+
+// this device's copy of scratch, looked up once with the device current
+int scratch_of(cpi_ctx *c, double **out) {
+  if (c->dev < 0 || c->dev >= kNoallocDevices)
+    return fail(c, CPI_ERR_INVALID, "cpi_stable_dt_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[c->dev]) {
+    HIPCHK(c, hipSetDevice(c->dev));
+    void *ptr = nullptr;
+    HIPCHK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[c->dev] = static_cast<double *>(ptr);
+  }
+  *out = g_scratch[c->dev];
+  return CPI_OK;
+}
+
+}  // namespace
+
+extern "C" int cpi_stable_dt_noalloc(cpi_ctx *c, double *dte, double *dtc) {
+  if (!c || !dte || !dtc) return CPI_ERR_INVALID;
+  if (!c->inited)
+    return fail(c, CPI_ERR_STATE, "cpi_stable_dt_noalloc: needs cpi_init (mass assembled)");
+  const int nb = (c->ne + 255) / 256;
+  if (nb > kNoallocPartials)
+    return fail(c, CPI_ERR_INVALID,
+                "cpi_stable_dt_noalloc: at most " + std::to_string(256L * kNoallocPartials) +
+                    " elements (the mesh has " + std::to_string(c->ne) + ")");
+  double *sp = nullptr;
+  int rc = scratch_of(c, &sp);
+  if (rc) return rc;
+  HIPCHK(c, launch_elem_dt(c->P, sp, c->stream));
+  hipLaunchKernelGGL(k_dt_finish, dim3(1), dim3(256), 0, c->stream, (const double *)sp, nb,
+                     (const double *)c->P.m, c->nn, sp + kNoallocPartials);
+  HIPCHK(c, hipGetLastError());
+  double r[kNoallocOut];
+  HIPCHK(c, hipMemcpyAsync(r, sp + kNoallocPartials, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *dte = r[0];
+  *dtc = (c->wall.enabled && c->wall.stiffness > 0.0) ? 2.0 * std::sqrt(r[1] / c->wall.stiffness)
+                                                      : INFINITY;
+  return CPI_OK;
 }
 
 extern "C" int cpi_init(cpi_ctx *c, double dt) {

@@ -295,6 +295,14 @@ class Dynamics:
         L.check(L.lib().cpi_stable_dt(self.ctx, C.byref(a), C.byref(b)), self.ctx)
         return a.value, b.value
 
+    def stable_dt_noalloc(self):
+        """stable_dt of the current configuration without any allocation
+        (cpi_stable_dt_noalloc; needs init)."""
+        self._create()
+        a, b = C.c_double(), C.c_double()
+        L.check(L.lib().cpi_stable_dt_noalloc(self.ctx, C.byref(a), C.byref(b)), self.ctx)
+        return a.value, b.value
+
     def time_steps(self, n):
         ms = C.c_double()
         L.check(L.lib().cpi_time_steps(self.ctx, int(n), C.byref(ms)), self.ctx)

@@ -199,6 +199,18 @@ int cpi_set_state(cpi_ctx *ctx, const double *x0, const double *v0);
    it if cpi_init has not run. */
 int cpi_stable_dt(cpi_ctx *ctx, double *dt_element, double *dt_contact);
 
+/* cpi_stable_dt of the current configuration for callers that cannot
+   allocate (serving loops and harnesses that forbid allocation and check
+   the stable step between cpi_step calls; it waits for the context's
+   stream, so it is not a call to capture).  No counterpart in the
+   reference.
+   It calls no hipMalloc / hipFree and no allocating routine: its partial
+   minima go to a file-scope device array of 65,538 doubles, one per GPU,
+   held until the process ends.  Needs cpi_init (the mass); meshes above
+   16,777,216 elements are refused with CPI_ERR_INVALID.  Calls that overlap
+   on one GPU share that array and race.  Same results as cpi_stable_dt. */
+int cpi_stable_dt_noalloc(cpi_ctx *ctx, double *dt_element, double *dt_contact);
+
 /* First half of Dynamics::solve (dynamics.h:439-466): x = X0, mass assembly,
    the initial update with du = 0, and the half-kick v += dt/2 a0.
    dt <= 0 selects dt = -dt * dt_stable (e.g. -0.5 gives half the stable
