@@ -38,6 +38,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0   # dense MFMA
 MLP_FLOP = 2 * (13 * 64 + 64 * 64 + 64 * 7)  # 10,752 per evaluation
+# VALU issue cycles of one 16-row tile of k_mlp_queue, counted in the
+# kernel's gfx950 assembly: 32 v_exp_f32 + 32 v_rcp_f32 (16 cycles per
+# wave64) and 201 other VALU instructions (4 cycles); 1,024 SIMDs, 2.4 GHz
+MLP_TILE_VALU_CYCLES = 32 * 16 + 32 * 16 + 201 * 4
+SIMDS, CLOCK_HZ = 1024, 2.4e9
 
 
 def parse():
@@ -53,6 +58,12 @@ def parse():
     ap.add_argument("--speed", type=float, default=10.0, help="headline impact speed, m/s")
     ap.add_argument("--n", type=int, default=64, help="hexes per side (default 64)")
     ap.add_argument("--no-c4", action="store_true", help="skip the surrogate-at-work runs")
+    ap.add_argument("--c4-save-net", default="", help="write the trained c4 network here (.npz)")
+    ap.add_argument("--c4-load-net", default="",
+                    help="use this saved c4 network instead of collecting and training")
+    ap.add_argument("--only-c4", action="store_true",
+                    help="run only the surrogate-at-work runs and print their JSON object "
+                         "(for kernel profiles of the surrogate step)")
     ap.add_argument("--c4-speeds", default="20,30")
     ap.add_argument("--c4-target", type=int, default=10_000_000,
                     help="training samples (about)")
@@ -196,27 +207,33 @@ def c4_leg(args, ctx, x0, dt):
     from cppimpact_amd import workloads as W
     from cppimpact_amd.surrogate import Surrogate, workload_training_set
 
-    t0 = time.time()
-    smp, tinfo = workload_training_set([W.c1(), W.c2(args.n), W.c3()],
-                                       per_workload=args.c4_per_workload, target=args.c4_target,
-                                       seed=0, log=log)
-    t1 = time.time()
-    net = Surrogate(W.material(), seed=0)
-    hist = net.train(smp, epochs=args.c4_epochs, batch=1024)
-    t2 = time.time()
-    log(f"c4 network: {len(smp)} samples ({t1 - t0:.1f} s), {args.c4_epochs} epochs "
-        f"({t2 - t1:.1f} s), loss {hist[0]:.3g} -> {hist[-1]:.3g}")
+    if args.c4_load_net:
+        net = Surrogate.load(W.material(), args.c4_load_net)
+        training = {"loaded_from": args.c4_load_net}
+    else:
+        t0 = time.time()
+        smp, tinfo = workload_training_set([W.c1(), W.c2(args.n), W.c3()],
+                                           per_workload=args.c4_per_workload,
+                                           target=args.c4_target, seed=0, log=log)
+        t1 = time.time()
+        net = Surrogate(W.material(), seed=0)
+        hist = net.train(smp, epochs=args.c4_epochs, batch=1024)
+        t2 = time.time()
+        log(f"c4 network: {len(smp)} samples ({t1 - t0:.1f} s), {args.c4_epochs} epochs "
+            f"({t2 - t1:.1f} s), loss {hist[0]:.3g} -> {hist[-1]:.3g}")
+        training = dict(tinfo, epochs=args.c4_epochs, loss_first=float(hist[0]),
+                        loss_last=float(hist[-1]), collect_s=round(t1 - t0, 1),
+                        train_s=round(t2 - t1, 1),
+                        mesh_note="runs: " + ", ".join(f"{r['workload']}@{r['speed']}"
+                                                        for r in tinfo["runs"]))
+        if args.c4_save_net:
+            net.save(args.c4_save_net)
     ctx["mlp"].set_surrogate(net)
     steps = 500
     out = {"workload": f"c2 mesh ({x0.shape[0]} nodes), J2 aluminium, impact along -z, "
                        f"{steps} steps from rest at the wall, dt=0.5*stable, "
                        f"{args.reps} runs each timed whole",
-           "training": dict(tinfo, epochs=args.c4_epochs, loss_first=float(hist[0]),
-                            loss_last=float(hist[-1]), collect_s=round(t1 - t0, 1),
-                            train_s=round(t2 - t1, 1),
-                            mesh_note="runs: " + ", ".join(f"{r['workload']}@{r['speed']}"
-                                                            for r in tinfo["runs"])),
-           "tolerance": 1e-3, "speeds": {}}
+           "training": training, "tolerance": 1e-3, "speeds": {}}
     for v in [float(s) for s in args.c4_speeds.split(",")]:
         row = {}
         res = {}
@@ -262,6 +279,13 @@ def c4_leg(args, ctx, x0, dt):
                         "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 5),
                         "traffic": None, "flop_per_evaluation": MLP_FLOP,
                         "kernel_ms": round(prof["mlp_queue"], 5)}
+                    # the kernel's real bound: the activations' VALU issue
+                    floor = pts / 16 * MLP_TILE_VALU_CYCLES / SIMDS / CLOCK_HZ * 1e3
+                    row[name]["valu_bound_mlp_queue"] = {
+                        "bound": "valu (SiLU: exp + rcp per activation)",
+                        "floor_ms": round(floor, 5), "kernel_ms": round(prof["mlp_queue"], 5),
+                        "frac": round(floor / prof["mlp_queue"], 4),
+                        "cycles_per_tile": MLP_TILE_VALU_CYCLES}
             log(f"c4 {v:g} m/s {name}: {row[name]['value']:.4g} el-steps/s")
         for name in ("fp32", "mlp"):
             dmx, dl2 = rel_errors(res[name]["x"], res["fp64"]["x"], x0)
@@ -417,6 +441,13 @@ def main():
            "fp32": wl.dynamics(speed=args.speed, precision=L.PREC_F32, scatter=sc),
            "mlp": wl.dynamics(speed=args.speed, precision=L.PREC_F32,
                               stress_update=L.RETURN_MLP)}
+    if args.only_c4:
+        ctx["fp64"].init(0.0)
+        c4 = c4_leg(args, ctx, x, ctx["fp64"].info().dt)
+        for d in ctx.values():
+            d.close()
+        print(json.dumps({"c4": c4}), flush=True)
+        return
     plain = run_headline(ctx["fp64"], args, 0.0, L.PREC_F64, steps)
     dt = plain["dt"]
     log(f"fp64 {plain['value']:.4g} el-steps/s, {plain['ms_per_step']:.4f} ms/step")

@@ -574,7 +574,10 @@ k_elem_block4_walk(Params p) {
 
 // The network on queue rows [0, n): n from the step's counter (ctr) or
 // given.  Each workgroup with a tile to do copies the weights into LDS;
-// its waves take tiles w, w + waves, ...
+// its waves take tiles w, w + waves, ..., each reading the next tile's
+// inputs before it runs the network on the current one, so the row loads
+// (rows the previous launch wrote, from another XCD's L2 at best) overlap
+// the MFMA chain.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_mlp_queue(const MlpParams *P, uint16_t *rows,
                                                   const unsigned long long *ctr, int n_given) {
@@ -584,10 +587,16 @@ __global__ void __launch_bounds__(NT) k_mlp_queue(const MlpParams *P, uint16_t *
   const int ntiles = (int)((n + 15u) >> 4);
   if ((int)blockIdx.x * WPB >= ntiles) return;
   mlp_lds_load(P->weights, &W);
+  const int stride = (int)gridDim.x * WPB;
+  int tile = (int)blockIdx.x * WPB + (int)(threadIdx.x >> 6);
+  bf16x8 xb = mlp_tile_inputs(rows + (size_t)(tile < ntiles ? tile : 0) * 16 * kMlpRowHalves);
   __syncthreads();
-  for (int tile = (int)blockIdx.x * WPB + (int)(threadIdx.x >> 6); tile < ntiles;
-       tile += (int)gridDim.x * WPB)
-    mlp_tile(W, rows + (size_t)tile * 16 * kMlpRowHalves);
+  for (; tile < ntiles; tile += stride) {
+    const int next = tile + stride < ntiles ? tile + stride : tile;
+    const bf16x8 xn = mlp_tile_inputs(rows + (size_t)next * 16 * kMlpRowHalves);
+    mlp_tile(W, xb, rows + (size_t)tile * 16 * kMlpRowHalves);
+    xb = xn;
+  }
 }
 
 template <bool AUX, typename R>
@@ -1123,8 +1132,9 @@ template <typename R, int PHYS> constexpr int tet4_waves() {
 constexpr int kMlpWalkWaves = 6;
 
 // workgroups of k_mlp_queue in the step (four waves each, every wave
-// looping over tiles)
-constexpr int kMlpQueueGrid = 1024;
+// looping over tiles): eight per CU, eight waves per SIMD at 32 VGPRs and
+// 15 KB of LDS each
+constexpr int kMlpQueueGrid = 2048;
 constexpr int kMlpQueueThreads = 256;
 
 // LDS of the workgroup element kernels: node records, corner forces, the

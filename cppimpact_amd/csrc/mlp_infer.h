@@ -27,20 +27,24 @@ __device__ __forceinline__ void mlp_lds_load(const MlpWeights *src, MlpWeights *
   for (int i = threadIdx.x; i < (int)(sizeof(MlpWeights) / 16); i += blockDim.x) d[i] = s[i];
 }
 
-// The network on one tile of 16 queue rows (rows[16 * kMlpRowHalves]): row
-// n holds sample n's standardized BF16 inputs and receives its 7 FP32
-// outputs.  All 64 lanes of the wave take part.  Every product is computed
-// transposed (H^T = W X^T, see mlp.h), so lane n + 16g holds units
-// 4g..4g+3 of each 16-unit output tile of sample n, already the layout of
-// the next layer's B operand.  Row n is read by lanes n and n + 16 and
-// written by the same two lanes after the MFMAs consumed what they read.
-__device__ __forceinline__ void mlp_tile(const MlpWeights &W, uint16_t *rows) {
+// The layer-1 B operand of one tile of 16 queue rows for this lane: lane
+// n + 16g (g < 2) takes features 8g..8g+7 of row n, lanes with g >= 2 the
+// zero padding of K = 32.
+__device__ __forceinline__ bf16x8 mlp_tile_inputs(const uint16_t *rows) {
   const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
   const float zf[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const bf16x8 zero8 = pack_bf8(zf);
-  // layer-1 B operand: features 8g..8g+7 of sample n (g >= 2: the zero
-  // padding of K = 32)
-  const bf16x8 xb = g < 2 ? ld_bf8(rows + n * kMlpRowHalves + 8 * g) : zero8;
+  return g < 2 ? ld_bf8(rows + n * kMlpRowHalves + 8 * g) : pack_bf8(zf);
+}
+
+// The network on one tile of 16 queue rows (rows[16 * kMlpRowHalves]),
+// whose inputs xb were read by mlp_tile_inputs: row n holds sample n's
+// standardized BF16 inputs and receives its 7 FP32 outputs.  All 64 lanes
+// of the wave take part.  Every product is computed transposed (H^T = W
+// X^T, see mlp.h), so lane n + 16g holds units 4g..4g+3 of each 16-unit
+// output tile of sample n, already the layout of the next layer's B
+// operand.  Row n is written by lanes n and n + 16, the lanes that read it.
+__device__ __forceinline__ void mlp_tile(const MlpWeights &W, const bf16x8 xb, uint16_t *rows) {
+  const int l = threadIdx.x & 63, n = l & 15, g = l >> 4;
   // the scheduling barriers keep each group's LDS weight reads next to
   // their MFMAs: hoisted together the 14 weight fragments hold 56 VGPRs
   bf16x8 hb[2];

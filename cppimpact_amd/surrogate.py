@@ -52,6 +52,21 @@ class Surrogate:
                 (in_mean, in_std, out_mean, out_std)]
         self._chk(L.lib().cpi_mlp_set_scaling(self.handle, *(L.fptr(v) for v in arrs)))
 
+    def save(self, path):
+        """Writes the parameters and the scaling to an .npz file."""
+        a, b, c, d = self.scaling()
+        np.savez(path, params=self.params, in_mean=a, in_std=b, out_mean=c, out_std=d)
+
+    @classmethod
+    def load(cls, material, path, device=0):
+        """A network for material with the parameters and scaling of a file
+        save() wrote."""
+        f = np.load(path)
+        net = cls(material, device=device)
+        net.set_scaling(f["in_mean"], f["in_std"], f["out_mean"], f["out_std"])
+        net.params = f["params"]
+        return net
+
     def fit_scaling(self, samples):
         s = np.ascontiguousarray(samples, dtype=np.float32)
         self._chk(L.lib().cpi_mlp_fit_scaling(self.handle, L.fptr(s), len(s)))

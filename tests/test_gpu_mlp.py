@@ -285,6 +285,25 @@ def test_collected_set_and_training_are_reproducible():
     assert np.array_equal(params[0], params[1])
 
 
+def test_saved_network_loads_with_the_same_outputs(tmp_path):
+    """save() then load(): the same parameters and scaling, bit for bit, and
+    the same forward outputs."""
+    rng = np.random.default_rng(61)
+    a = Surrogate(material(), seed=5)
+    a.params = random_theta(rng)
+    a.set_scaling(rng.normal(size=L.MLP_INPUTS), rng.uniform(0.5, 2.0, L.MLP_INPUTS),
+                  rng.normal(size=L.MLP_OUTPUTS), rng.uniform(0.5, 2.0, L.MLP_OUTPUTS))
+    a.save(tmp_path / "net.npz")
+    b = Surrogate.load(material(), tmp_path / "net.npz")
+    assert np.array_equal(a.params, b.params)
+    for u, v in zip(a.scaling(), b.scaling()):
+        assert np.array_equal(u, v)
+    x = rng.normal(size=(100, M.N_IN)).astype(np.float32)
+    assert np.array_equal(a.forward(x), b.forward(x))
+    a.close()
+    b.close()
+
+
 def _zero_network(out_mean):
     """A network whose every output is exactly out_mean: all weights and
     biases zero (SiLU(0) = 0), unit output scale."""
