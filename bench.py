@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--c4-save-net", default="", help="write the trained c4 network here (.npz)")
     ap.add_argument("--c4-load-net", default="",
                     help="use this saved c4 network instead of collecting and training")
+    ap.add_argument("--only-c3", action="store_true",
+                    help="run only the cylinder runs and print their JSON object (for kernel "
+                         "profiles and counters of the tet10 kernel)")
     ap.add_argument("--only-c4", action="store_true",
                     help="run only the surrogate-at-work runs and print their JSON object "
                          "(for kernel profiles of the surrogate step)")
@@ -431,6 +434,9 @@ def main():
     gbs = ctypes.c_double(0.0)
     L.check(L.lib().cpi_bandwidth_probe(0, 1 << 30, 20, ctypes.byref(gbs)))
     log(f"stream copy {gbs.value:.0f} GB/s")
+    if args.only_c3:
+        print(json.dumps({"cylinder_c3": c3_summary(args)}), flush=True)
+        return
     wl = W.c2(args.n)
     x = wl.x
     log(f"mesh {wl.elements} tet4, {x.shape[0]} nodes")

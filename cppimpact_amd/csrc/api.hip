@@ -448,10 +448,14 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     if (mnl < 4L * npe)
       return fail(c, CPI_ERR_INVALID, "block_elements " + std::to_string(me) +
                                           " does not fit the LDS budget");
-    int mn = (int)std::min<long>(mnl, npe == 4 ? nt : 1024);
+    // the tet4 and the per-point tet10 kernels give each thread at most one
+    // node; the five-point tet10 kernel loops over its nodes
+    int mn = (int)std::min<long>(mnl, (npe == 4 || c->nq == 4) ? nt : 1024);
     std::string e;
     if (!build_block_plan(c->ne, npe, c->conn_u.data(), c->nn, c->X0u.data(), me, mn, c->plan, e))
       return fail(c, CPI_ERR_INVALID, "partition: " + e);
+    if (c->plan.max_local_nodes > mn)
+      return fail(c, CPI_ERR_INVALID, "partition: a block holds more nodes than its kernel");
     c->eperm = c->plan.elem_perm;
     c->nperm = c->plan.node_perm;
     c->ninv = c->plan.node_inv;

@@ -395,97 +395,224 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 }
 
 // tet10 on a 4-point rule with one lane per point: lanes 4k..4k+3 hold the
-// same element e, lane q = (lane & 3) point q.  The point's kinematics and
-// stress are the arithmetic of tet10_points for that point; each corner
-// force w detJ sigma . dN_i is then summed over the quad's lanes as
-// (q0 + q1) + (q2 + q3) (quad_sum, DPP inside the quad), so every lane
-// holds corner i's element force and hands it to put(i, f).  All four
-// lanes of a quad must be active.  pre: the point's stress state, loaded
-// ahead (EP models).
+// same element e, lane q = (lane & 3) point q.  All four lanes of a quad
+// must be active.  pre: the point's stress state, loaded ahead (EP models).
+//
+// The basis gradients of tet10_grad at a point, with a = 4 xi, b = 4 eta,
+// c = 4 zeta and l = 4 (1 - xi - eta - zeta), are
+//   G_0 = (a-1, 0, 0)    G_1 = (0, b-1, 0)    G_2 = (1-l)(1, 1, 1)
+//   G_3 = (0, 0, c-1)    G_4 = (b, a, 0)      G_5 = (-b, l-b, -b)
+//   G_6 = (l-a, -a, -a)  G_7 = (c, 0, a)      G_8 = (0, c, b)
+//   G_9 = (-c, -c, l-c)
+// and the point's arithmetic is written on that structure:
+//   J  = sum_i (x_i - x_2) (x) G_i          (node 2 is the origin)
+//   Lv = sum_i v_i (x) G_i, grad v = Lv J^-1, de = dt sym(grad v)
+//   M  = w detJ sigma J^-T, f_i = M G_i      (= w detJ sigma . dN_i)
+// so a node's force is one or two products per component instead of the
+// gradient dN_i = J^-T G_i and sigma . dN_i.  Each corner force is then
+// summed over the quad's lanes as (q0 + q1) + (q2 + q3) (quad_sum, DPP
+// inside the quad), so every lane holds corner i's element force and hands
+// it to put(i, f).
 template <int RULE, typename R, int PHYS, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
                                                  EAcc &ea) {
   using Q = Quad<RULE>;
   static_assert(Q::n == 4, "one lane per point of a 4-point rule");
+  const double px = Q::pt(q, 0), py = Q::pt(q, 1), pz = Q::pt(q, 2);
+  const R a = R(4.0 * px), b = R(4.0 * py), c = R(4.0 * pz);
+  const R l = R(4.0 - 4.0 * px - 4.0 * py - 4.0 * pz);
+  // the differences are formed where used: held for the whole point they
+  // would take 14 more VGPRs in FP64
+  auto am1 = [&]() { return a - R(1); };
+  auto bm1 = [&]() { return b - R(1); };
+  auto cm1 = [&]() { return c - R(1); };
+  auto l1 = [&]() { return R(1) - l; };
+  auto la = [&]() { return l - a; };
+  auto lb = [&]() { return l - b; };
+  auto lc = [&]() { return l - c; };
   double o[3];
   {
     double ov[3];
     get(2, o, ov);
   }
-  // the point's basis gradients in the element precision
-  R Gd[30];
-  tet10_grad<R>(R(Q::pt(q, 0)), R(Q::pt(q, 1)), R(Q::pt(q, 2)), Gd);
-  R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-#pragma unroll
-  for (int i = 0; i < 10; i++) {
+  auto rel = [&](int i, R d[3]) {
     double xi[3], vi[3];
     get(i, xi, vi);
-    const R d[3] = {R(xi[0] - o[0]), R(xi[1] - o[1]), R(xi[2] - o[2])};
-    tet10_jac_add<R>(J, Gd, i, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) d[r] = R(xi[r] - o[r]);
+  };
+  auto vel = [&](int i, R u[3]) {
+    double xi[3], vi[3];
+    get(i, xi, vi);
+#pragma unroll
+    for (int r = 0; r < 3; r++) u[r] = R(vi[r]);
+  };
+  // J[r][k] = d x_r / d xi_k; the stable dt of the corner tetrahedron
+  // (nodes 0, 1, 3 about node 2) from the same differences
+  R J[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+  {
+    R d[3], e1[3], e2[3];
+    rel(0, e1);
+#pragma unroll
+    for (int r = 0; r < 3; r++) J[3 * r] += am1() * e1[r];
+    rel(1, e2);
+#pragma unroll
+    for (int r = 0; r < 3; r++) J[3 * r + 1] += bm1() * e2[r];
+    rel(3, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) J[3 * r + 2] += cm1() * d[r];
+    if constexpr (PHYS != PHYS_COMPAT) {
+      R v6 = e1[0] * (e2[1] * d[2] - e2[2] * d[1]) - e1[1] * (e2[0] * d[2] - e2[2] * d[0]) +
+             e1[2] * (e2[0] * d[1] - e2[1] * d[0]);
+      ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, d, v6, p.wave_speed, kTet10LengthFactor));
+    }
+    rel(4, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] += b * d[r];
+      J[3 * r + 1] += a * d[r];
+    }
+    rel(5, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] -= b * d[r];
+      J[3 * r + 1] += lb() * d[r];
+      J[3 * r + 2] -= b * d[r];
+    }
+    rel(6, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] += la() * d[r];
+      J[3 * r + 1] -= a * d[r];
+      J[3 * r + 2] -= a * d[r];
+    }
+    rel(7, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] += c * d[r];
+      J[3 * r + 2] += a * d[r];
+    }
+    rel(8, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r + 1] += c * d[r];
+      J[3 * r + 2] += b * d[r];
+    }
+    rel(9, d);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      J[3 * r] -= c * d[r];
+      J[3 * r + 1] -= c * d[r];
+      J[3 * r + 2] += lc() * d[r];
+    }
   }
   R Ji[9];
-  R detJ = inv3<R>(J, Ji);
+  const R detJ = inv3<R>(J, Ji);
   if (!(detJ > R(0))) ea.bad += 1.0;
-  // strain rate sum_i dN_i (x) v_i, times dt once at the end
-  R de[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+  // Lv[r][k] = d v_r / d xi_k
+  R Lv[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+  {
+    R u[3];
+    vel(0, u);
 #pragma unroll
-  for (int i = 0; i < 10; i++) {
-    double xi[3], vi[3];
-    get(i, xi, vi);
-    R u[3] = {R(vi[0]), R(vi[1]), R(vi[2])};
-    R dn[3];
-    tet10_dn<R>(Ji, Gd, i, dn);
-    de[0] += dn[0] * u[0];
-    de[1] += dn[1] * u[1];
-    de[2] += dn[2] * u[2];
-    de[3] += dn[1] * u[0] + dn[0] * u[1];
-    de[4] += dn[2] * u[1] + dn[1] * u[2];
-    de[5] += dn[2] * u[0] + dn[0] * u[2];
+    for (int r = 0; r < 3; r++) Lv[3 * r] += am1() * u[r];
+    vel(1, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) Lv[3 * r + 1] += bm1() * u[r];
+    vel(2, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] += l1() * u[r];
+      Lv[3 * r + 1] += l1() * u[r];
+      Lv[3 * r + 2] += l1() * u[r];
+    }
+    vel(3, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) Lv[3 * r + 2] += cm1() * u[r];
+    vel(4, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] += b * u[r];
+      Lv[3 * r + 1] += a * u[r];
+    }
+    vel(5, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] -= b * u[r];
+      Lv[3 * r + 1] += lb() * u[r];
+      Lv[3 * r + 2] -= b * u[r];
+    }
+    vel(6, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] += la() * u[r];
+      Lv[3 * r + 1] -= a * u[r];
+      Lv[3 * r + 2] -= a * u[r];
+    }
+    vel(7, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] += c * u[r];
+      Lv[3 * r + 2] += a * u[r];
+    }
+    vel(8, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r + 1] += c * u[r];
+      Lv[3 * r + 2] += b * u[r];
+    }
+    vel(9, u);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      Lv[3 * r] -= c * u[r];
+      Lv[3 * r + 1] -= c * u[r];
+      Lv[3 * r + 2] += lc() * u[r];
+    }
   }
+  // grad v[r][s] = sum_k Lv[r][k] Ji[k][s]; engineering shear
+  auto gv = [&](int r, int s) {
+    return Lv[3 * r] * Ji[s] + Lv[3 * r + 1] * Ji[3 + s] + Lv[3 * r + 2] * Ji[6 + s];
+  };
+  R de[6] = {gv(0, 0), gv(1, 1), gv(2, 2), gv(0, 1) + gv(1, 0), gv(1, 2) + gv(2, 1),
+             gv(0, 2) + gv(2, 0)};
 #pragma unroll
-  for (int c = 0; c < 6; c++) de[c] *= R(dt);
+  for (int k = 0; k < 6; k++) de[k] *= R(dt);
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
   point_stress<R, PHYS, true>(p, e, q, de, wd, s, ea, &pre);
+  // M[r][k] = sum_t (w detJ sigma)[r][t] Ji[k][t], sigma rows (s0 s3 s5),
+  // (s3 s1 s4), (s5 s4 s2)
+  R M[9], S[3];
+  {
+    R w[6];
 #pragma unroll
-  for (int i = 0; i < 10; i++) {
-    R dn[3], g[3], f[3];
-    if constexpr (std::is_same<R, double>::value) {
-      // FP64: node i's gradients pass through an empty asm so the strain
-      // pass's dN values are recomputed here, not kept live across the
-      // stress update (60 VGPRs; with them the elastoplastic kernel spills)
-      R Gi[30];
+    for (int k = 0; k < 6; k++) w[k] = wd * s[k];
 #pragma unroll
-      for (int k = 0; k < 30; k++) Gi[k] = Gd[k];
-#pragma unroll
-      for (int k = 3 * i; k < 3 * i + 3; k++)
-        if (tet10_nz(k)) __asm__ volatile("" : "+v"(Gi[k]));
-      tet10_dn<R>(Ji, Gi, i, dn);
-    } else {
-      tet10_dn<R>(Ji, Gd, i, dn);
+    for (int k = 0; k < 3; k++) {
+      M[k] = w[0] * Ji[3 * k] + w[3] * Ji[3 * k + 1] + w[5] * Ji[3 * k + 2];
+      M[3 + k] = w[3] * Ji[3 * k] + w[1] * Ji[3 * k + 1] + w[4] * Ji[3 * k + 2];
+      M[6 + k] = w[5] * Ji[3 * k] + w[4] * Ji[3 * k + 1] + w[2] * Ji[3 * k + 2];
     }
-    sig_dn<R>(dn, s, g);
 #pragma unroll
-    for (int c = 0; c < 3; c++) f[c] = quad_sum(wd * g[c]);
+    for (int r = 0; r < 3; r++) S[r] = M[3 * r] + M[3 * r + 1] + M[3 * r + 2];
+  }
+  // f_i = M G_i (f_5, f_6, f_9 through S = M (1, 1, 1)), summed over the
+  // element's points
+  auto emit = [&](int i, R f0, R f1, R f2) {
+    const R f[3] = {quad_sum(f0), quad_sum(f1), quad_sum(f2)};
     put(i, f);
-  }
-  if constexpr (PHYS != PHYS_COMPAT) {
-    R e1[3], e2[3], e3[3];
-    double xa[3], va[3];
-    get(0, xa, va);
-#pragma unroll
-    for (int c = 0; c < 3; c++) e1[c] = R(xa[c] - o[c]);
-    get(1, xa, va);
-#pragma unroll
-    for (int c = 0; c < 3; c++) e2[c] = R(xa[c] - o[c]);
-    get(3, xa, va);
-#pragma unroll
-    for (int c = 0; c < 3; c++) e3[c] = R(xa[c] - o[c]);
-    R v6 = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
-           e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
-    ea.mindt = fmin(ea.mindt, corner_dt<R>(e1, e2, e3, v6, p.wave_speed, kTet10LengthFactor));
-  }
+  };
+  emit(0, am1() * M[0], am1() * M[3], am1() * M[6]);
+  emit(1, bm1() * M[1], bm1() * M[4], bm1() * M[7]);
+  emit(2, l1() * S[0], l1() * S[1], l1() * S[2]);
+  emit(3, cm1() * M[2], cm1() * M[5], cm1() * M[8]);
+  emit(4, b * M[0] + a * M[1], b * M[3] + a * M[4], b * M[6] + a * M[7]);
+  emit(5, l * M[1] - b * S[0], l * M[4] - b * S[1], l * M[7] - b * S[2]);
+  emit(6, l * M[0] - a * S[0], l * M[3] - a * S[1], l * M[6] - a * S[2]);
+  emit(7, c * M[0] + a * M[2], c * M[3] + a * M[5], c * M[6] + a * M[8]);
+  emit(8, c * M[1] + b * M[2], c * M[4] + b * M[5], c * M[7] + b * M[8]);
+  emit(9, l * M[2] - c * S[0], l * M[5] - c * S[1], l * M[8] - c * S[2]);
 }
 
 // tet10 with the corner forces summed in registers, handed to put() at the end

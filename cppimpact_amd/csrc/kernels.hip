@@ -295,47 +295,64 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 
 // tet10 on a 4-point rule: four lanes per element, lane t & 3 taking
 // quadrature point t & 3, so a workgroup of NT threads holds at most NT/4
-// elements.  Each point's stress state is loaded before the first barrier.
-// Corner k's element force (summed over the element's four lanes in rule
-// order) is stored by lane k & 3 into the corner's jagged-diagonal slot;
-// the nodes are then assembled as in k_elem_block.  LDS as k_elem_block.
+// elements, and the plan gives it at most NT local nodes, one per thread.
+// As in the tet4 kernel every global load of the workgroup (the point's
+// stress state, the element's local node ids and corner slots, the node's
+// id, slot and contribution count, the node records) is issued before the
+// first barrier; after it the element pass reads only LDS and the node pass
+// only LDS and its registers.  Corner k's element force (summed over the
+// element's four lanes in rule order) is stored by lane k & 3 into the
+// corner's jagged-diagonal slot; the nodes are then assembled as in
+// k_elem_block.  LDS as k_elem_block.
 template <int RULE, typename R, int PHYS, bool AUX, int NT, int WPE>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block10(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int t = threadIdx.x, q = t & 3;
+  int g = -1, slot = -1, deg = 0;
+  if (t < p.ml) {
+    const long o = (long)b * p.ml + t;
+    g = p.nid_pad[o];
+    slot = p.slot_pad[o];
+    deg = p.deg_pad[o];
+  }
   const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
   const int32_t *mt = p.meta + 8 * b;
   const int es = mt[0], nel = mt[1];
-  const int ml = p.ml;
-  const int32_t *nid = p.nid_pad + (long)b * ml;
-  const LNodes ln{smem, ml};
-  R *lf = (R *)((char *)smem + lds_node_bytes(ml));
+  const LNodes ln{smem, p.ml};
+  R *lf = (R *)((char *)smem + lds_node_bytes(p.ml));
   uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 10, p.max_el));
   const double dt = p.ctl->dt;
   const bool has_e = (t >> 2) < nel;
   const int e = es + (t >> 2);
+  // the element's ten local node ids and corner slots, two per word (the
+  // rows of 20 bytes are 4-byte aligned)
+  uint32_t lcw[5] = {0u, 0u, 0u, 0u, 0u}, psw[5] = {0u, 0u, 0u, 0u, 0u};
   PreState<R> pre;
-  if constexpr (PHYS != PHYS_COMPAT) {
-    if (has_e) load_state<R>(p, e, q, pre.s, pre.ep);
+  if (has_e) {
+    const uint32_t *lc = reinterpret_cast<const uint32_t *>(p.conn_local + 10L * e);
+    const uint32_t *ps = reinterpret_cast<const uint32_t *>(p.csr_pos + 10L * e);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+      lcw[k] = lc[k];
+      psw[k] = ps[k];
+    }
+    if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, q, pre.s, pre.ep);
   }
-
-  for (int j = t; j < ml; j += NT) {
-    const int g = nid[j];
-    if (g >= 0) ln.stage(j, p.xm[g], p.vv[g]);
-  }
+  if (g >= 0) ln.stage(t, p.xm[g], p.vv[g]);
   for (int k = t; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
   __syncthreads();
 
   EAcc ea;
   if (has_e) {
-    const uint16_t *lc = p.conn_local + (long)e * 10;
-    const uint16_t *pos = p.csr_pos + (long)e * 10;
-    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(lc[k], xk, vk); };
+    auto half = [](const uint32_t (&w)[5], int k) {
+      return (int)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
+    };
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(half(lcw, k), xk, vk); };
     auto put = [&](int k, const R f[3]) {
       if ((k & 3) == q) {
-        R *dst = lf + 3 * pos[k];
+        R *dst = lf + 3 * half(psw, k);
         dst[0] = f[0];
         dst[1] = f[1];
         dst[2] = f[2];
@@ -345,17 +362,13 @@ k_elem_block10(Params p) {
   }
   __syncthreads();
 
-  for (int j = t; j < ml; j += NT) {
-    const int g = nid[j];
-    if (g < 0) continue;
-    const int deg = p.deg_pad[(long)b * ml + j];
+  if (g >= 0) {
     double f[3];
-    node_force_sum<R>(lf, joff, j, deg, f);
-    const int slot = p.slot_pad[(long)b * ml + j];
+    node_force_sum<R>(lf, joff, t, deg, f);
     if (slot < 0) {
       double x[3], v[3];
-      ln.get(j, x, v);
-      node_update<AUX>(p, g, f, x, v, ln.mass(j), dt, ea);
+      ln.get(t, x, v);
+      node_update<AUX>(p, g, f, x, v, ln.mass(t), dt, ea);
     } else {
       store_partial<R>(p, slot, f);
     }
