@@ -583,11 +583,12 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     P.n_boundary = B.n_boundary;
     P.max_loc = B.max_local_nodes;
     P.max_el = B.max_block_elems;
-    // the yielded bits: tet4, exact return (the surrogate kernel reads ep
-    // always and keeps no bits)
-    if (npe == 4 && c->phys == CPI_PHYSICS_EP) {
+    // the yielded bits, one per stress point: tet4 and the per-point tet10
+    // kernel, exact return (the surrogate kernel reads ep always and keeps
+    // no bits)
+    if ((npe == 4 || c->nq == 4) && c->phys == CPI_PHYSICS_EP) {
       unsigned long long *yw = nullptr;
-      P.ywords = (B.max_block_elems + 63) / 64;
+      P.ywords = (c->nq * B.max_block_elems + 63) / 64;
       if ((rc = dalloc(c, &yw, (size_t)P.ywords * B.nblocks))) return rc;
       P.yielded = yw;
     }

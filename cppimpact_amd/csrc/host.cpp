@@ -10,6 +10,7 @@
 #include <fstream>
 #include <map>
 #include <numeric>
+#include <queue>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -126,6 +127,7 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
     }
   double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
   std::vector<uint64_t> key(ne);
+  std::vector<double> cen(3 * (size_t)ne);
   const double scale = double((1u << 21) - 1) / span;
   for (int e = 0; e < ne; e++) {
     double c[3] = {0, 0, 0};
@@ -133,45 +135,157 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
       for (int k = 0; k < 3; k++) c[k] += xloc[3 * conn[(size_t)npe * e + j] + k];
     uint32_t q[3];
     for (int k = 0; k < 3; k++) {
+      cen[3 * (size_t)e + k] = 0.25 * c[k];
       double v = (0.25 * c[k] - lo[k]) * scale;
       v = std::min(std::max(v, 0.0), double((1u << 21) - 1));
       q[k] = (uint32_t)v;
     }
     key[e] = hilbert3(q[0], q[1], q[2]);
   }
-  std::vector<int32_t> order(ne);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
+  std::vector<int32_t> horder(ne);
+  std::iota(horder.begin(), horder.end(), 0);
+  std::stable_sort(horder.begin(), horder.end(),
                    [&](int a, int b) { return key[a] < key[b]; });
 
-  // 2. greedy chunking under both limits
+  // 2. two partitions under both limits, the one with fewer partial slots
+  // (the sum over shared nodes of the blocks that hold them) kept: the
+  // Hilbert order cut greedily into blocks, which gives the octree cells of
+  // a regular grid, and blocks grown from seeds taken in Hilbert order,
+  // each adding the candidate element that shares the most nodes with it
+  // (ties: the one nearest the seed), which follows a thin shell through
+  // its thickness where the Hilbert cuts split it.
   std::vector<int32_t> stamp(nn, -1);
-  std::vector<int32_t> blk_start;  // into order
-  int cur = -1, cur_elems = 0, cur_nodes = 0;
-  for (int t = 0; t < ne; t++) {
-    const int32_t *c = conn + (size_t)npe * order[t];
+  auto count_fresh = [&](int e, int cur) {
+    const int32_t *c = conn + (size_t)npe * e;
     int fresh = 0;
-    if (cur >= 0)
-      for (int j = 0; j < npe; j++) {
-        bool dup = false;
-        for (int k = 0; k < j; k++) dup |= (c[k] == c[j]);
-        if (!dup && stamp[c[j]] != cur) fresh++;
-      }
-    if (cur < 0 || cur_elems + 1 > max_elems || cur_nodes + fresh > max_nodes) {
-      cur++;
-      blk_start.push_back(t);
-      cur_elems = 0;
-      cur_nodes = 0;
+    for (int j = 0; j < npe; j++) {
+      bool dup = false;
+      for (int k = 0; k < j; k++) dup |= (c[k] == c[j]);
+      if (!dup && stamp[c[j]] != cur) fresh++;
     }
-    for (int j = 0; j < npe; j++)
-      if (stamp[c[j]] != cur) {
-        stamp[c[j]] = cur;
-        cur_nodes++;
+    return fresh;
+  };
+  auto slots_of = [&](const std::vector<int32_t> &ord, const std::vector<int32_t> &bs) {
+    std::fill(stamp.begin(), stamp.end(), -1);
+    std::vector<int32_t> cnt(nn, 0);
+    for (int b = 0; b + 1 < (int)bs.size(); b++)
+      for (int t = bs[b]; t < bs[b + 1]; t++)
+        for (int j = 0; j < npe; j++) {
+          const int g = conn[(size_t)npe * ord[t] + j];
+          if (stamp[g] != b) {
+            stamp[g] = b;
+            cnt[g]++;
+          }
+        }
+    long s = 0;
+    for (int i = 0; i < nn; i++)
+      if (cnt[i] > 1) s += cnt[i];
+    return s;
+  };
+  // 2a. greedy chunking of the Hilbert order
+  std::vector<int32_t> hstart;
+  {
+    std::fill(stamp.begin(), stamp.end(), -1);
+    int cur = -1, cur_elems = 0, cur_nodes = 0;
+    for (int t = 0; t < ne; t++) {
+      const int e = horder[t];
+      const int fresh = cur >= 0 ? count_fresh(e, cur) : 0;
+      if (cur < 0 || cur_elems + 1 > max_elems || cur_nodes + fresh > max_nodes) {
+        cur++;
+        hstart.push_back(t);
+        cur_elems = 0;
+        cur_nodes = 0;
       }
-    cur_elems++;
+      const int32_t *c = conn + (size_t)npe * e;
+      for (int j = 0; j < npe; j++)
+        if (stamp[c[j]] != cur) {
+          stamp[c[j]] = cur;
+          cur_nodes++;
+        }
+      cur_elems++;
+    }
+    hstart.push_back(ne);
   }
-  const int nb = (int)blk_start.size();
-  blk_start.push_back(ne);
+  // 2b. grown blocks
+  std::vector<int32_t> gorder, gstart;
+  {
+    std::vector<int32_t> nptr(nn + 1, 0), nel;
+    for (size_t k = 0; k < (size_t)ne * npe; k++) nptr[conn[k] + 1]++;
+    for (int i = 0; i < nn; i++) nptr[i + 1] += nptr[i];
+    nel.resize(nptr[nn]);
+    {
+      std::vector<int32_t> fill(nptr.begin(), nptr.end() - 1);
+      for (int e = 0; e < ne; e++)
+        for (int j = 0; j < npe; j++) nel[fill[conn[(size_t)npe * e + j]]++] = e;
+    }
+    std::fill(stamp.begin(), stamp.end(), -1);
+    std::vector<char> done(ne, 0);
+    std::vector<int32_t> shared(ne, 0), touched;
+    gorder.reserve(ne);
+    typedef std::pair<std::pair<int, double>, int> Ent;  // (shared, -dist^2), element
+    int seedp = 0, cur = 0;
+    while ((int)gorder.size() < ne) {
+      while (done[horder[seedp]]) seedp++;
+      const int seed = horder[seedp];
+      const double *sc = &cen[3 * (size_t)seed];
+      auto dist = [&](int f) {
+        const double *fc = &cen[3 * (size_t)f];
+        return (fc[0] - sc[0]) * (fc[0] - sc[0]) + (fc[1] - sc[1]) * (fc[1] - sc[1]) +
+               (fc[2] - sc[2]) * (fc[2] - sc[2]);
+      };
+      gstart.push_back((int)gorder.size());
+      std::priority_queue<Ent> pq;
+      int nodes = 0, els = 0;
+      touched.clear();
+      auto add = [&](int e) {
+        done[e] = 1;
+        gorder.push_back(e);
+        els++;
+        for (int j = 0; j < npe; j++) {
+          const int g = conn[(size_t)npe * e + j];
+          if (stamp[g] == cur) continue;
+          stamp[g] = cur;
+          nodes++;
+          for (int k = nptr[g]; k < nptr[g + 1]; k++) {
+            const int f = nel[k];
+            if (done[f]) continue;
+            if (shared[f]++ == 0) touched.push_back(f);
+            pq.push(Ent({shared[f], -dist(f)}, f));
+          }
+        }
+      };
+      add(seed);
+      while (els < max_elems) {
+        int pick = -1;
+        while (!pq.empty()) {
+          const Ent &top = pq.top();
+          if (done[top.second] || top.first.first != shared[top.second]) {
+            pq.pop();
+            continue;
+          }
+          pick = top.second;
+          break;
+        }
+        if (pick < 0) {
+          // no neighbour left: the next element of the Hilbert order
+          int sp = seedp;
+          while (sp < ne && done[horder[sp]]) sp++;
+          if (sp >= ne) break;
+          pick = horder[sp];
+        }
+        if (nodes + count_fresh(pick, cur) > max_nodes) break;
+        add(pick);
+      }
+      for (int f : touched) shared[f] = 0;
+      cur++;
+    }
+    gstart.push_back(ne);
+  }
+  const bool grown = slots_of(gorder, gstart) < slots_of(horder, hstart);
+  P.grown = grown;
+  const std::vector<int32_t> &order = grown ? gorder : horder;
+  std::vector<int32_t> blk_start = grown ? gstart : hstart;
+  const int nb = (int)blk_start.size() - 1;
 
   // 3. per node: number of referencing blocks and the first one
   std::vector<int32_t> nblk(nn, 0), first(nn, -1);

@@ -32,6 +32,7 @@ struct BlockPlan {
   int n_interior = 0;       // device ids [0, n_interior) are interior nodes
   int n_boundary = 0;       // device ids [n_interior, n_interior+n_boundary)
   int n_slots = 0;
+  bool grown = false;       // grown blocks (else the cut Hilbert order)
   std::vector<int32_t> elem_perm;  // device element -> user element
   std::vector<int32_t> node_perm;  // device node -> user node
   std::vector<int32_t> node_inv;   // user node -> device node

@@ -297,7 +297,8 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 // quadrature point t & 3, so a workgroup of NT threads holds at most NT/4
 // elements, and the plan gives it at most NT local nodes, one per thread.
 // As in the tet4 kernel every global load of the workgroup (the point's
-// stress state, the element's local node ids and corner slots, the node's
+// stress state, its ep only when the point's yielded bit says it has
+// yielded, the element's local node ids and corner slots, the node's
 // id, slot and contribution count, the node records) is issued before the
 // first barrier; after it the element pass reads only LDS and the node pass
 // only LDS and its registers.  Corner k's element force (summed over the
@@ -330,6 +331,13 @@ k_elem_block10(Params p) {
   // rows of 20 bytes are 4-byte aligned)
   uint32_t lcw[5] = {0u, 0u, 0u, 0u, 0u}, psw[5] = {0u, 0u, 0u, 0u, 0u};
   PreState<R> pre;
+  // yielded bits, one per lane (point) of the wave: a point's ep is read
+  // only when it has yielded before
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  const bool has_word = PHYS == PHYS_EP && p.yielded && wv < p.ywords;
+  const long yi = (long)b * p.ywords + wv;
+  unsigned long long yw = ~0ull;
+  if (has_word) yw = p.yielded[yi];
   if (has_e) {
     const uint32_t *lc = reinterpret_cast<const uint32_t *>(p.conn_local + 10L * e);
     const uint32_t *ps = reinterpret_cast<const uint32_t *>(p.csr_pos + 10L * e);
@@ -338,7 +346,7 @@ k_elem_block10(Params p) {
       lcw[k] = lc[k];
       psw[k] = ps[k];
     }
-    if constexpr (PHYS != PHYS_COMPAT) load_state<R>(p, e, q, pre.s, pre.ep);
+    if constexpr (PHYS != PHYS_COMPAT) load_state_if<R>(p, e, q, (yw >> (t & 63)) & 1ull, pre);
   }
   if (g >= 0) ln.stage(t, p.xm[g], p.vv[g]);
   for (int k = t; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
@@ -359,6 +367,10 @@ k_elem_block10(Params p) {
       }
     };
     tet10_point_lane<RULE, R, PHYS>(p, get, dt, e, q, pre, put, ea);
+  }
+  if constexpr (PHYS == PHYS_EP) {
+    const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
+    if (has_word && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
   __syncthreads();
 

@@ -142,8 +142,9 @@ static void check(const std::string &label, int ne, int npe, const std::vector<i
   }
   for (int d = 0; d < P.n_interior; d++)
     REQUIRE(blocks_of_node[d] == 1, "interior node %d in %d blocks", d, blocks_of_node[d]);
-  std::printf("ok %s: %d elements, %d blocks, %d interior + %d boundary nodes, %d slots\n",
-              label.c_str(), ne, nb, P.n_interior, P.n_boundary, P.n_slots);
+  std::printf("ok %s: %d elements, %d %s blocks, %d interior + %d boundary nodes, %d slots\n",
+              label.c_str(), ne, nb, P.grown ? "grown" : "Hilbert", P.n_interior, P.n_boundary,
+              P.n_slots);
 }
 
 static void box(const std::string &label, int n, int quad, double jitter, int max_el,

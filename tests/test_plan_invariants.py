@@ -26,3 +26,6 @@ def test_block_plan_invariants_under_sanitizers(tmp_path):
     assert p.returncode == 0, p.stdout + p.stderr
     assert "all plans hold" in p.stdout
     assert p.stdout.count("ok ") == 8
+    # both partitions are exercised: the cut Hilbert order on the boxes, the
+    # grown blocks on the thin cylinder
+    assert " Hilbert blocks" in p.stdout and " grown blocks" in p.stdout
