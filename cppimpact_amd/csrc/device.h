@@ -69,10 +69,10 @@ struct Params {
   const int32_t *conn;   // [npe][ne] device node ids (colour / atomic)
   void *sig;             // [nq][6][ne] stress state (ep)
   void *epbar;           // [nq][ne]
-  // tet4 block kernel, exact return: one bit per element, set once it has
-  // yielded (ep > 0), ywords 64-bit words per block (element es + t of
-  // block b at word b*ywords + t/64, bit t%64); ep of the others is 0 and
-  // is not read.  Null: ep is always read.
+  // tet4 and per-point tet10 block kernels, exact return: one bit per
+  // stress point, set once it has yielded (ep > 0), ywords 64-bit words per
+  // block (the point of thread t of block b at word b*ywords + t/64, bit
+  // t%64); ep of the others is 0 and is not read.  Null: ep is always read.
   unsigned long long *yielded;
   int ywords;
   // workgroup plan (LDS-assembled scatter)
