@@ -252,3 +252,49 @@ def test_create_refuses_an_inverted_element():
     with pytest.raises(L.CpiError) as ex:
         d.init(0.0)
     assert ex.value.code == -4 and "element 5 is inverted" in str(ex.value)
+
+
+@pytest.mark.parametrize("what", ["no elements", "no nodes", "npe 7", "physics 5", "scatter 9",
+                                  "E 0", "nu 0.5", "wall dim 3", "wall normal 0",
+                                  "surrogate on tet10", "surrogate in FP64"])
+def test_create_rejects_invalid_descriptions_before_the_device(what):
+    """cpi_create validates the mesh, options, material and wall before it
+    touches a device, so each bad description is CPI_ERR_INVALID with a
+    message here too (the reference has no such checks)."""
+    lib = L.lib()
+    quad = what == "surrogate on tet10"
+    x, conn = ci.box_mesh(2, 2, 2, 0.1, 0.1, 0.1, quadratic=quad)
+    x = np.ascontiguousarray(x, np.float64)
+    conn = np.ascontiguousarray(conn, np.int32)
+    desc = L.MeshDesc(x.shape[0], conn.shape[0], conn.shape[1], L.dptr(x), L.iptr(conn), 0,
+                      None, 0, None)
+    mat = L.Material(68.9e9, 2700.0, 0.33, 1.0, 700e6, 276e6)
+    wall = L.WallDesc(1, 2, 1, 0.0, 1e9)
+    opts = {}
+    if what == "no elements":
+        desc.num_elements = 0
+    elif what == "no nodes":
+        desc.num_nodes = 0
+    elif what == "npe 7":
+        desc.nodes_per_element = 7
+    elif what == "physics 5":
+        opts["physics"] = 5
+    elif what == "scatter 9":
+        opts["scatter"] = 9
+    elif what == "E 0":
+        mat.E = 0.0
+    elif what == "nu 0.5":
+        mat.nu = 0.5
+    elif what == "wall dim 3":
+        wall.dim = 3
+    elif what == "wall normal 0":
+        wall.normal = 0
+    elif what == "surrogate on tet10":
+        opts.update(physics=L.PHYSICS_EP, precision=L.PREC_F32, stress_update=L.RETURN_MLP)
+    elif what == "surrogate in FP64":
+        opts.update(physics=L.PHYSICS_EP, stress_update=L.RETURN_MLP)
+    o = L.default_options(**opts)
+    ctx = C.c_void_p()
+    rc = lib.cpi_create(C.byref(desc), C.byref(mat), C.byref(wall), C.byref(o), C.byref(ctx))
+    assert rc == -1, (what, rc)
+    assert lib.cpi_last_error(None), what
