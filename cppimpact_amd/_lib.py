@@ -57,7 +57,7 @@ class Options(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "physics", "precision", "scatter", "quadrature", "mass", "stress_update",
         "use_graph", "track_energy", "contact_slave_only", "block_elements", "device",
-        "fixed_bc", "debug_fill_nan")] + [("gravity", C.c_double * 3),
+        "fixed_bc", "debug_fill_nan", "stream_state")] + [("gravity", C.c_double * 3),
                                           ("mlp_tolerance", C.c_double)]
 
 

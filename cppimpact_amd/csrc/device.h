@@ -75,6 +75,9 @@ struct Params {
   // t%64); ep of the others is 0 and is not read.  Null: ep is always read.
   unsigned long long *yielded;
   int ywords;
+  // 1: the per-point tet10 kernel reads and writes the stress state with
+  // non-temporal accesses (a state larger than the last-level cache)
+  int stream_state;
   // workgroup plan (LDS-assembled scatter)
   const int32_t *meta;
   const uint16_t *conn_local;

@@ -45,27 +45,46 @@ template <> struct Quad<RULE_CENTROID> {
   __device__ static constexpr double pt(int, int) { return 0.25; }
 };
 
-template <typename R>
+// Stress-state words are read and written once per step.  STREAM = true
+// marks those accesses non-temporal: for a state larger than the
+// last-level cache, caching it only evicts the node records and partial
+// forces that other blocks and the boundary-node kernel read again.
+template <bool STREAM, typename T>
+__device__ __forceinline__ T state_ld(const T *a) {
+  if constexpr (STREAM)
+    return __builtin_nontemporal_load(a);
+  else
+    return *a;
+}
+template <bool STREAM, typename T>
+__device__ __forceinline__ void state_st(T *a, T v) {
+  if constexpr (STREAM)
+    __builtin_nontemporal_store(v, a);
+  else
+    *a = v;
+}
+
+template <typename R, bool STREAM = false>
 __device__ __forceinline__ void load_state(const Params &p, int e, int q, R s[6], R &ep) {
   const R *S = (const R *)p.sig;
   const R *P = (const R *)p.epbar;
   const long ne = p.ne;
 #pragma unroll
-  for (int c = 0; c < 6; c++) s[c] = S[(long)(q * 6 + c) * ne + e];
-  ep = P[(long)q * ne + e];
+  for (int c = 0; c < 6; c++) s[c] = state_ld<STREAM>(&S[(long)(q * 6 + c) * ne + e]);
+  ep = state_ld<STREAM>(&P[(long)q * ne + e]);
 }
 
 // ep is written only when it changed (a plastic step): elastic steps store
 // the stress alone
-template <typename R>
+template <typename R, bool STREAM = false>
 __device__ __forceinline__ void store_state(const Params &p, int e, int q, const R s[6], R ep,
                                             bool ep_changed = true) {
   R *S = (R *)p.sig;
   R *P = (R *)p.epbar;
   const long ne = p.ne;
 #pragma unroll
-  for (int c = 0; c < 6; c++) S[(long)(q * 6 + c) * ne + e] = s[c];
-  if (ep_changed) P[(long)q * ne + e] = ep;
+  for (int c = 0; c < 6; c++) state_st<STREAM>(&S[(long)(q * 6 + c) * ne + e], s[c]);
+  if (ep_changed) state_st<STREAM>(&P[(long)q * ne + e], ep);
 }
 
 // State of one point loaded ahead of use (the tet4 workgroup kernel issues
@@ -78,14 +97,14 @@ template <typename R> struct PreState {
 
 // The stress of one point, and its ep only when the point has yielded
 // before (ep of the others is 0).
-template <typename R>
+template <typename R, bool STREAM = false>
 __device__ __forceinline__ void load_state_if(const Params &p, int e, int q, bool has_ep,
                                               PreState<R> &st) {
   const R *S = (const R *)p.sig;
   const long ne = p.ne;
 #pragma unroll
-  for (int c = 0; c < 6; c++) st.s[c] = S[(long)(q * 6 + c) * ne + e];
-  st.ep = has_ep ? ((const R *)p.epbar)[(long)q * ne + e] : R(0);
+  for (int c = 0; c < 6; c++) st.s[c] = state_ld<STREAM>(&S[(long)(q * 6 + c) * ne + e]);
+  st.ep = has_ep ? state_ld<STREAM>(&((const R *)p.epbar)[(long)q * ne + e]) : R(0);
 }
 
 // Stress at one point from the strain increment de.  compat: sigma = D de
@@ -93,7 +112,7 @@ __device__ __forceinline__ void load_state_if(const Params &p, int e, int q, boo
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
 // point's internal work and plastic dissipation (times wd) to ea.  (The
 // surrogate's return map is applied by the tet4 block kernel's walk.)
-template <typename R, int PHYS, bool PRE = false>
+template <typename R, int PHYS, bool PRE = false, bool STREAM = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
                                              PreState<R> *pre = nullptr) {
@@ -138,7 +157,7 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-    store_state<R>(p, e, q, s, ep, dg > R(0));
+    store_state<R, STREAM>(p, e, q, s, ep, dg > R(0));
     if constexpr (PRE) pre->ep = ep;
   }
 }
@@ -413,7 +432,7 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 // summed over the quad's lanes as (q0 + q1) + (q2 + q3) (quad_sum, DPP
 // inside the quad), so every lane holds corner i's element force and hands
 // it to put(i, f).
-template <int RULE, typename R, int PHYS, class Get, class Put>
+template <int RULE, typename R, int PHYS, bool STREAM, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
                                                  EAcc &ea) {
@@ -580,7 +599,7 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   for (int k = 0; k < 6; k++) de[k] *= R(dt);
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
-  point_stress<R, PHYS, true>(p, e, q, de, wd, s, ea, &pre);
+  point_stress<R, PHYS, true, STREAM>(p, e, q, de, wd, s, ea, &pre);
   // M[r][k] = sum_t (w detJ sigma)[r][t] Ji[k][t], sigma rows (s0 s3 s5),
   // (s3 s1 s4), (s5 s4 s2)
   R M[9], S[3];

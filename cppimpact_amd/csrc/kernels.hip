@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 // element's four lanes in rule order) is stored by lane k & 3 into the
 // corner's jagged-diagonal slot; the nodes are then assembled as in
 // k_elem_block.  LDS as k_elem_block.
-template <int RULE, typename R, int PHYS, bool AUX, int NT, int WPE>
+template <int RULE, typename R, int PHYS, bool AUX, int NT, int WPE, bool STREAM>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block10(Params p) {
   extern __shared__ double smem[];
@@ -346,7 +346,8 @@ k_elem_block10(Params p) {
       lcw[k] = lc[k];
       psw[k] = ps[k];
     }
-    if constexpr (PHYS != PHYS_COMPAT) load_state_if<R>(p, e, q, (yw >> (t & 63)) & 1ull, pre);
+    if constexpr (PHYS != PHYS_COMPAT)
+      load_state_if<R, STREAM>(p, e, q, (yw >> (t & 63)) & 1ull, pre);
   }
   if (g >= 0) ln.stage(t, p.xm[g], p.vv[g]);
   for (int k = t; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
@@ -366,7 +367,7 @@ k_elem_block10(Params p) {
         dst[2] = f[2];
       }
     };
-    tet10_point_lane<RULE, R, PHYS>(p, get, dt, e, q, pre, put, ea);
+    tet10_point_lane<RULE, R, PHYS, STREAM>(p, get, dt, e, q, pre, put, ea);
   }
   if constexpr (PHYS == PHYS_EP) {
     const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
@@ -1222,8 +1223,15 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
     CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
   } else if constexpr (Quad<RULE>::n == 4) {
     constexpr int W = tet10_waves<R>();
-    if (aux) CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
-    CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
+    if constexpr (PHYS != PHYS_COMPAT) {
+      if (p.stream_state) {
+        if (aux)
+          CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, true, NT, W, true>), nblocks, NT, smem, s, p);
+        CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, false, NT, W, true>), nblocks, NT, smem, s, p);
+      }
+    }
+    if (aux) CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, true, NT, W, false>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block10<RULE, R, PHYS, false, NT, W, false>), nblocks, NT, smem, s, p);
   } else {
     if (aux) CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, true, NT>), nblocks, NT, smem, s, p);
     CPI_LAUNCH((k_elem_block<NPE, RULE, R, PHYS, false, NT>), nblocks, NT, smem, s, p);
@@ -1303,9 +1311,13 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
         return lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
       } else if constexpr (Quad<RULE>::n == 4) {
         constexpr int W = tet10_waves<R>();
-        hipError_t e = lds_opt_in(k_elem_block10<RULE, R, PHYS, true, NT, W>, smem);
-        if (e != hipSuccess) return e;
-        return lds_opt_in(k_elem_block10<RULE, R, PHYS, false, NT, W>, smem);
+        hipError_t e = lds_opt_in(k_elem_block10<RULE, R, PHYS, true, NT, W, false>, smem);
+        if (e == hipSuccess) e = lds_opt_in(k_elem_block10<RULE, R, PHYS, false, NT, W, false>, smem);
+        if constexpr (PHYS != PHYS_COMPAT) {
+          if (e == hipSuccess) e = lds_opt_in(k_elem_block10<RULE, R, PHYS, true, NT, W, true>, smem);
+          if (e == hipSuccess) e = lds_opt_in(k_elem_block10<RULE, R, PHYS, false, NT, W, true>, smem);
+        }
+        return e;
       } else {
         hipError_t e = lds_opt_in(k_elem_block<NPE, RULE, R, PHYS, true, NT>, smem);
         if (e != hipSuccess) return e;

@@ -139,6 +139,30 @@ def test_tet10_block_size_option():
     ref["dyn"].close()
 
 
+@pytest.mark.parametrize("precision", [L.PREC_F64, L.PREC_F32])
+def test_tet10_streamed_state_gives_the_same_bits(precision):
+    """Non-temporal stress-state accesses (chosen by size for large states,
+    forced here on a small mesh) change where the state is cached, not its
+    values: the same bits as cached accesses over a plastic impact, and the
+    oracle's state in FP64."""
+    x, conn = cube(5, quadratic=True, jitter=0.05, seed=4, gap=1e-6)
+    k = wall_k(0.25, 10)
+    v0, dt, n = (0, 0, -40.0), 1e-7, 50
+    runs = [run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=L.QUAD_EXACT4,
+                    precision=precision, stream_state=s) for s in (1, 2)]
+    for key in ("x", "v", "a", "energy"):
+        assert np.array_equal(runs[0][key], runs[1][key]), key
+    s1, e1 = runs[0]["dyn"].element_state()
+    s2, e2 = runs[1]["dyn"].element_state()
+    assert e1.max() > 0.0
+    assert np.array_equal(s1, s2) and np.array_equal(e1, e2)
+    if precision == L.PREC_F64:
+        o = run_oracle(x, conn, v0, dt, n, O.EP, k, quad_rule=L.QUAD_EXACT4)
+        assert _disp_err(runs[0], o, x) < 1e-9 and rel(runs[0]["v"], o["v"]) < 1e-9
+    for g in runs:
+        g["dyn"].close()
+
+
 def test_f32_elements_within_1e3_of_fp64_oracle():
     x, conn = cube(8, gap=1e-6, jitter=0.1, seed=3)
     k = wall_k(0.25, 8)

@@ -256,7 +256,7 @@ def test_create_refuses_an_inverted_element():
 
 @pytest.mark.parametrize("what", ["no elements", "no nodes", "npe 7", "physics 5", "scatter 9",
                                   "E 0", "nu 0.5", "wall dim 3", "wall normal 0",
-                                  "surrogate on tet10", "surrogate in FP64"])
+                                  "surrogate on tet10", "surrogate in FP64", "stream_state 3"])
 def test_create_rejects_invalid_descriptions_before_the_device(what):
     """cpi_create validates the mesh, options, material and wall before it
     touches a device, so each bad description is CPI_ERR_INVALID with a
@@ -293,6 +293,8 @@ def test_create_rejects_invalid_descriptions_before_the_device(what):
         opts.update(physics=L.PHYSICS_EP, precision=L.PREC_F32, stress_update=L.RETURN_MLP)
     elif what == "surrogate in FP64":
         opts.update(physics=L.PHYSICS_EP, stress_update=L.RETURN_MLP)
+    elif what == "stream_state 3":
+        opts["stream_state"] = 3
     o = L.default_options(**opts)
     ctx = C.c_void_p()
     rc = lib.cpi_create(C.byref(desc), C.byref(mat), C.byref(wall), C.byref(o), C.byref(ctx))
