@@ -431,7 +431,7 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 // gradient dN_i = J^-T G_i and sigma . dN_i.  Each corner force is then
 // summed over the quad's lanes as (q0 + q1) + (q2 + q3) (quad_sum, DPP
 // inside the quad), so every lane holds corner i's element force and hands
-// it to put(i, f).
+// it to put(i, f).  STREAM: the point's state is stored non-temporal.
 template <int RULE, typename R, int PHYS, bool STREAM, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
@@ -441,8 +441,8 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   const double px = Q::pt(q, 0), py = Q::pt(q, 1), pz = Q::pt(q, 2);
   const R a = R(4.0 * px), b = R(4.0 * py), c = R(4.0 * pz);
   const R l = R(4.0 - 4.0 * px - 4.0 * py - 4.0 * pz);
-  // the differences are formed where used: held for the whole point they
-  // would take 14 more VGPRs in FP64
+  // the differences are formed where they are used, not held for the
+  // whole point
   auto am1 = [&]() { return a - R(1); };
   auto bm1 = [&]() { return b - R(1); };
   auto cm1 = [&]() { return c - R(1); };
