@@ -152,9 +152,11 @@ def run_headline(dyn, args, dt, precision, steps):
 def roofline(r, traffic_key):
     """The element pass against the HBM roofline: algorithmic bytes per
     launch (formula_elem_bytes) over its mean duration from HIP events on
-    the context's stream.  In the surrogate mode the element pass is three
-    launches (elastic pass, network queue, deferred-block walk), priced
-    together (their sum with no event between them, "elem_block")."""
+    the context's stream, recorded by the kernel launches themselves at the
+    kernel's start and end (cpi_profile_kernels).  In the surrogate mode the
+    element pass is three launches (elastic pass, network queue,
+    deferred-block walk), priced together ("elem_block": from the start of
+    the first to the end of the third)."""
     prof = r["prof"]
     kname = [k for k in prof if k.startswith("elem_block") or k.startswith("elem_direct")][0]
     ms = prof[kname]

@@ -1076,41 +1076,61 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
   // with the surrogate the element pass is three launches: every other step
   // times them one by one, "elem_pass" (the elastic pass), "mlp_queue" (the
   // network on the queue) and "mlp_walk" (the deferred blocks), and the
-  // steps between time the three with no event between them ("elem_block";
-  // an event there adds a few microseconds)
+  // steps between time the three together ("elem_block": from the start of
+  // the first to the end of the last).  Block-scatter kernels are timed by
+  // events their launch records at the kernel's own start and end
+  // (hipExtLaunchKernel); the colour and atomic scatters' launches by events
+  // recorded around them.
   const bool mlp = c->scatter == CPI_SCATTER_BLOCK && c->phys == 2;
   const char *names[5] = {c->scatter == CPI_SCATTER_BLOCK ? "elem_block" : "elem_direct",
                           c->scatter == CPI_SCATTER_BLOCK ? "node_bnd" : "node_all", "elem_pass",
                           "mlp_queue", "mlp_walk"};
   double tot[5] = {0, 0, 0, 0, 0};
   int64_t launches[5] = {0, 0, 0, 0, 0};
-  // ev[0] step start, ev[1] element pass end, ev[2] node pass end,
-  // ev[3], ev[4] after the first and second surrogate launch
-  hipEvent_t ev[5];
-  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  // ev[0], ev[1] element pass start and end; ev[2], ev[3] node pass; ev[4..9]
+  // the three surrogate launches one by one (destroyed on every return, and
+  // no launch keeps them)
+  struct Events {
+    hipEvent_t e[10] = {};
+    ~Events() {
+      set_launch_events(nullptr, nullptr);
+      for (auto &x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  hipEvent_t *ev = evs.e;
+  for (int k = 0; k < 10; k++) HIPCHK(c, hipEventCreate(&ev[k]));
+  auto elapsed = [&](int a, int b, float &ms) -> int {
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return CPI_OK;
+  };
   for (int i = 0; i < nsteps; i++) {
     Params p = c->P;
     p.store_aux = 0;
     const bool split = mlp && i % 2 == 0;
-    HIPCHK(c, hipEventRecord(ev[0], c->stream));
     hipError_t e = hipSuccess;
-    if (split) {
-      p.epart_base = 0;
-      e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream, 1);
-      HIPCHK(c, hipEventRecord(ev[3], c->stream));
-      if (e == hipSuccess)
+    p.epart_base = 0;
+    if (mlp) {
+      for (int part = 1; part <= 3 && e == hipSuccess; part++) {
+        if (split)
+          set_launch_events(ev[2 + 2 * part], ev[3 + 2 * part]);
+        else
+          set_launch_events(part == 1 ? ev[0] : nullptr, part == 3 ? ev[1] : nullptr);
         e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream,
-                              2);
-      HIPCHK(c, hipEventRecord(ev[4], c->stream));
-      if (e == hipSuccess)
-        e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream,
-                              3);
-      for (int k = 2; k < 5; k++) launches[k] += 1;
+                              part);
+      }
+      set_launch_events(nullptr, nullptr);
+      if (split)
+        for (int k = 2; k < 5; k++) launches[k] += 1;
+      else
+        launches[0] += 1;
     } else if (c->scatter == CPI_SCATTER_BLOCK) {
-      p.epart_base = 0;
+      set_launch_events(ev[0], ev[1]);
       e = launch_elem_block(p, c->rule, c->prec, c->phys, false, c->nblocks, c->smem, c->stream);
+      set_launch_events(nullptr, nullptr);
       launches[0] += 1;
     } else {
+      HIPCHK(c, hipEventRecord(ev[0], c->stream));
       const bool atomic = c->scatter == CPI_SCATTER_ATOMIC;
       const int nc = atomic ? 1 : c->ncol;
       for (int k = 0; k < nc && e == hipSuccess; k++) {
@@ -1120,32 +1140,37 @@ extern "C" int cpi_profile_kernels(cpi_ctx *c, int32_t nsteps, cpi_kernel_time *
         e = launch_elem_direct(p, c->rule, c->prec, c->phys, atomic, c->stream);
         launches[0] += 1;
       }
+      HIPCHK(c, hipEventRecord(ev[1], c->stream));
     }
-    HIPCHK(c, hipEventRecord(ev[1], c->stream));
     p.epart_base = c->n_elem_rows;
-    if (e == hipSuccess)
-      e = c->scatter == CPI_SCATTER_BLOCK ? launch_node_bnd(p, c->prec, false, c->stream)
-                                          : launch_node_all(p, false, c->stream);
-    launches[1] += 1;
-    HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
-    HIPCHK(c, hipEventSynchronize(ev[2]));
-    float t0 = 0, t1 = 0;
-    HIPCHK(c, hipEventElapsedTime(&t0, ev[0], ev[1]));
-    HIPCHK(c, hipEventElapsedTime(&t1, ev[1], ev[2]));
-    if (!split) tot[0] += t0;
-    tot[1] += t1;
-    if (split) {
-      float ta = 0, tb = 0, tc = 0;
-      HIPCHK(c, hipEventElapsedTime(&ta, ev[0], ev[3]));
-      HIPCHK(c, hipEventElapsedTime(&tb, ev[3], ev[4]));
-      HIPCHK(c, hipEventElapsedTime(&tc, ev[4], ev[1]));
-      tot[2] += ta;
-      tot[3] += tb;
-      tot[4] += tc;
+    if (e == hipSuccess) {
+      if (c->scatter == CPI_SCATTER_BLOCK) {
+        set_launch_events(ev[2], ev[3]);
+        e = launch_node_bnd(p, c->prec, false, c->stream);
+        set_launch_events(nullptr, nullptr);
+      } else {
+        HIPCHK(c, hipEventRecord(ev[2], c->stream));
+        e = launch_node_all(p, false, c->stream);
+        HIPCHK(c, hipEventRecord(ev[3], c->stream));
+      }
     }
+    launches[1] += 1;
+    if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float t = 0;
+    int rc;
+    if (split) {
+      for (int k = 0; k < 3; k++) {
+        if ((rc = elapsed(4 + 2 * k, 5 + 2 * k, t))) return rc;
+        tot[2 + k] += t;
+      }
+    } else {
+      if ((rc = elapsed(0, 1, t))) return rc;
+      tot[0] += t;
+    }
+    if ((rc = elapsed(2, 3, t))) return rc;
+    tot[1] += t;
   }
-  for (auto &e : ev) (void)hipEventDestroy(e);
   c->step += nsteps;
   // one step only: the element pass is the sum of its launches
   if (launches[0] == 0 && launches[2] > 0) {

@@ -18,6 +18,10 @@ int elem_direct_threads(int npe);
 hipError_t launch_elem_block(const Params &p, int rule, int prec, int phys, bool aux,
                              int nblocks, size_t smem, hipStream_t s, int part = 0);
 hipError_t prepare_elem_block(int npe, int rule, int prec, int phys, size_t smem);
+// Events the following kernel launches of this host thread record at the
+// kernel's own start and end (hipExtLaunchKernel); (null, null) restores
+// plain launches.  For per-kernel timing outside graph capture.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 hipError_t launch_elem_direct(const Params &p, int rule, int prec, int phys, bool atomic,
                               hipStream_t s);
 // prec: the element kernel's precision (its partials are stored in it)

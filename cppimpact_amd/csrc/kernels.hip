@@ -18,6 +18,7 @@
 // are followed by k_node_all.
 //
 // Nodal state lives in 32-byte records: xm = (x, y, z, m), vv = (v, 0).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -1129,10 +1130,31 @@ __global__ void __launch_bounds__(256) k_stream_copy(const Node4 *__restrict__ s
 // launchers
 // ---------------------------------------------------------------------------
 
-#define CPI_LAUNCH(kern, grid, block, smem, stream, ...)                                 \
-  do {                                                                                   \
-    hipLaunchKernelGGL((kern), dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__);  \
-    return hipGetLastError();                                                            \
+// Events the next element or node kernel launch records at its own start
+// and end (hipExtLaunchKernel), for cpi_profile_kernels: the kernel's
+// execution time without the event packets' own latency.  Null: a plain
+// launch.  Per host thread.
+static thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+  t_ev_start = start;
+  t_ev_stop = stop;
+}
+
+template <typename K, typename... A>
+static inline void timed_launch(K kern, dim3 grid, dim3 block, size_t smem, hipStream_t s,
+                                A... args) {
+  if (t_ev_start || t_ev_stop)
+    hipExtLaunchKernelGGL(kern, grid, block, (std::uint32_t)smem, s, t_ev_start, t_ev_stop, 0,
+                          args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, smem, s, args...);
+}
+
+#define CPI_LAUNCH(kern, grid, block, smem, stream, ...)                      \
+  do {                                                                        \
+    timed_launch((kern), dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__); \
+    return hipGetLastError();                                                 \
   } while (0)
 
 // threads per workgroup of the block kernels: tet4 one per element (384),
@@ -1197,23 +1219,23 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
     constexpr int W = tet4_waves<R, PHYS>();
     if (part == 0 || part == 1) {
       if (aux)
-        hipLaunchKernelGGL((k_elem_block4<R, PHYS, true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+        timed_launch((k_elem_block4<R, PHYS, true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
       else
-        hipLaunchKernelGGL((k_elem_block4<R, PHYS, false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+        timed_launch((k_elem_block4<R, PHYS, false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
     }
     if (part == 0 || part == 2) {
       const int tiles = (p.ne + 15) / 16;
       const int grid = std::max(1, std::min(kMlpQueueGrid, (tiles + 3) / 4));
-      hipLaunchKernelGGL((k_mlp_queue<kMlpQueueThreads>), dim3(grid), dim3(kMlpQueueThreads), 0, s,
+      timed_launch((k_mlp_queue<kMlpQueueThreads>), dim3(grid), dim3(kMlpQueueThreads), 0, s,
                          p.mlp, p.mlp_rows, (const unsigned long long *)p.mlp_ctr, 0);
     }
     if (part == 0 || part == 3) {
       const int grid = nblocks;
       if (aux)
-        hipLaunchKernelGGL((k_elem_block4_walk<true, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
+        timed_launch((k_elem_block4_walk<true, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
                            smem, s, p);
       else
-        hipLaunchKernelGGL((k_elem_block4_walk<false, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
+        timed_launch((k_elem_block4_walk<false, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
                            smem, s, p);
     }
     return hipGetLastError();

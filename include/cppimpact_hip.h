@@ -270,12 +270,16 @@ int cpi_get_coloring(cpi_ctx *ctx, int32_t *color, int32_t *num_colors);
    returns the elapsed GPU time. */
 int cpi_time_steps(cpi_ctx *ctx, int32_t nsteps, double *elapsed_ms);
 
-/* Runs nsteps steps launching each kernel separately with HIP events
-   around every launch on the context's stream; fills per-kernel totals:
-   the element pass ("elem_block" / "elem_direct") and the node pass
-   ("node_bnd" / "node_all"), then, in the surrogate mode, the element
-   pass's three launches one by one on every other step ("elem_pass",
-   "mlp_queue", "mlp_walk").  *count = entries written (at most cap). */
+/* Runs nsteps steps launching each kernel separately on the context's
+   stream and times it with HIP events: block-scatter kernels by events
+   their launch records at the kernel's own start and end
+   (hipExtLaunchKernel), the colour and atomic scatters by events recorded
+   around their launches.  Fills per-kernel totals: the element pass
+   ("elem_block" / "elem_direct"; in the surrogate mode from the start of
+   its first launch to the end of its third) and the node pass ("node_bnd" /
+   "node_all"), then, in the surrogate mode, the element pass's three
+   launches one by one on every other step ("elem_pass", "mlp_queue",
+   "mlp_walk").  *count = entries written (at most cap). */
 int cpi_profile_kernels(cpi_ctx *ctx, int32_t nsteps, cpi_kernel_time *out,
                         int32_t cap, int32_t *count);
 
