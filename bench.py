@@ -139,10 +139,17 @@ def run_headline(dyn, args, dt, precision, steps):
     stats = dyn.surrogate_stats() if dyn.options.get("stress_update") else None
     prof = per_launch(dyn.profile_kernels(args.profile_steps))
     dyn.synchronize()
+    # points that took the surrogate's return during the profiled steps: none
+    # means no block was deferred, and the elastic pass did the whole pass
+    prof_yield = 0
+    if stats is not None:
+        after = dyn.surrogate_stats()
+        prof_yield = after[0] + after[1] - stats[0] - stats[1]
     med, lo, hi = stats_of(secs)
     ne = info.num_elements
     return dict(dt=info.dt, info=info, x=xg, ep=ep, energy=energy.tolist(), stats=stats,
-                prof=prof, value=ne * steps / med, value_min=ne * steps / hi,
+                prof=prof, prof_yield=prof_yield, value=ne * steps / med,
+                value_min=ne * steps / hi,
                 value_max=ne * steps / lo, ms_per_step=1e3 * med / steps,
                 elem_bytes=formula_elem_bytes(info, precision),
                 step_bytes=formula_step_bytes(info, precision),
@@ -150,15 +157,28 @@ def run_headline(dyn, args, dt, precision, steps):
 
 
 def roofline(r, traffic_key):
-    """The element pass against the HBM roofline: algorithmic bytes per
+    """The dominant kernel against the HBM roofline: algorithmic bytes per
     launch (formula_elem_bytes) over its mean duration from HIP events on
     the context's stream, recorded by the kernel launches themselves at the
     kernel's start and end (cpi_profile_kernels).  In the surrogate mode the
     element pass is three launches (elastic pass, network queue,
-    deferred-block walk), priced together ("elem_block": from the start of
-    the first to the end of the third)."""
+    deferred-block walk).  When no point yielded in the profiled steps no
+    block was deferred: the elastic pass did the element pass's whole
+    traffic and is the kernel priced ("elem_pass"), the other two found
+    empty lists; the three together are reported beside it
+    ("element_pass_three_launches").  Otherwise the three are priced
+    together ("elem_block": from the start of the first to the end of the
+    third)."""
     prof = r["prof"]
     kname = [k for k in prof if k.startswith("elem_block") or k.startswith("elem_direct")][0]
+    whole = None
+    if "elem_pass" in prof and r.get("prof_yield") == 0:
+        ms_all = prof[kname]
+        ach_all = r["elem_bytes"] / (ms_all * 1e-3) / 1e9
+        whole = {"kernel_ms": round(ms_all, 5), "achieved": round(ach_all, 1),
+                 "frac": round(ach_all / HBM_PEAK_GBS, 4)}
+        kname = "elem_pass"
+        traffic_key += "_elastic"
     ms = prof[kname]
     achieved = r["elem_bytes"] / (ms * 1e-3) / 1e9
     traffic = None
@@ -177,6 +197,9 @@ def roofline(r, traffic_key):
     for k in ("elem_pass", "mlp_queue", "mlp_walk"):
         if k in prof:
             out[k + "_ms"] = round(prof[k], 5)
+    if whole is not None:
+        out["element_pass_three_launches"] = whole
+        out["points_yielding_in_profiled_steps"] = 0
     return out
 
 
