@@ -107,6 +107,28 @@ __device__ __forceinline__ void load_state_if(const Params &p, int e, int q, boo
   st.ep = has_ep ? state_ld<STREAM>(&((const R *)p.epbar)[(long)q * ne + e]) : R(0);
 }
 
+// A zero word in global memory (the bits of 0.0 in FP64 and in FP32); no
+// kernel writes it.  Not const: a constant-memory address selected against
+// a global one makes the load a flat load, which the compiler waits for in
+// full before the next barrier.
+__device__ double kZeroWord[1] = {0.0};
+
+// load_state_if without a branch, for the block kernels' load phase: a
+// point that has not yielded loads its ep from kZeroWord, so every lane
+// issues the same seven loads, nothing waits for them before their first
+// use, and the kernel can wait for the loads issued before them by count
+// while these stay in flight.
+template <typename R, bool STREAM = false>
+__device__ __forceinline__ void load_state_sel(const Params &p, int e, int q, bool has_ep,
+                                               PreState<R> &st) {
+  const R *S = (const R *)p.sig;
+  const R *P = (const R *)p.epbar;
+  const long ne = p.ne;
+#pragma unroll
+  for (int c = 0; c < 6; c++) st.s[c] = state_ld<STREAM>(&S[(long)(q * 6 + c) * ne + e]);
+  st.ep = state_ld<STREAM>(has_ep ? &P[(long)q * ne + e] : (const R *)kZeroWord);
+}
+
 // Stress at one point from the strain increment de.  compat: sigma = D de
 // with no memory (dynamics_kernels.h:101-114 evaluates K_e du); ep: trial
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the

@@ -312,13 +312,6 @@ k_elem_block10(Params p) {
   extern __shared__ double smem[];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int t = threadIdx.x, q = t & 3;
-  int g = -1, slot = -1, deg = 0;
-  if (t < p.ml) {
-    const long o = (long)b * p.ml + t;
-    g = p.nid_pad[o];
-    slot = p.slot_pad[o];
-    deg = p.deg_pad[o];
-  }
   const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
   const int32_t *mt = p.meta + 8 * b;
   const int es = mt[0], nel = mt[1];
@@ -326,12 +319,6 @@ k_elem_block10(Params p) {
   R *lf = (R *)((char *)smem + lds_node_bytes(p.ml));
   uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 10, p.max_el));
   const double dt = p.ctl->dt;
-  const bool has_e = (t >> 2) < nel;
-  const int e = es + (t >> 2);
-  // the element's ten local node ids and corner slots, two per word (the
-  // rows of 20 bytes are 4-byte aligned)
-  uint32_t lcw[5] = {0u, 0u, 0u, 0u, 0u}, psw[5] = {0u, 0u, 0u, 0u, 0u};
-  PreState<R> pre;
   // yielded bits, one per lane (point) of the wave: a point's ep is read
   // only when it has yielded before
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -339,19 +326,36 @@ k_elem_block10(Params p) {
   const long yi = (long)b * p.ywords + wv;
   unsigned long long yw = ~0ull;
   if (has_word) yw = p.yielded[yi];
-  if (has_e) {
-    const uint32_t *lc = reinterpret_cast<const uint32_t *>(p.conn_local + 10L * e);
-    const uint32_t *ps = reinterpret_cast<const uint32_t *>(p.csr_pos + 10L * e);
+  // the loads in the order of the tet4 body (block4): node ids, then the
+  // node records and slot offsets, then connectivity and stress state, with
+  // no branch between them (one offset per thread: kmax <= max_el <= NT/4)
+  const long o = (long)b * p.ml + (t < p.ml ? t : p.ml - 1);
+  const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
+  const bool has_n = t < p.ml && g_o >= 0;
+  const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
+  const Node4 xr = p.xm[has_n ? g_o : 0], vr = p.vv[has_n ? g_o : 0];
+  const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
+  __builtin_amdgcn_sched_barrier(0);
+  const bool has_e = (t >> 2) < nel;
+  const int e = es + (t >> 2), ec = has_e ? e : es + nel - 1;
+  // the element's ten local node ids and corner slots, two per word (the
+  // rows of 20 bytes are 4-byte aligned)
+  uint32_t lcw[5], psw[5];
+  {
+    const uint32_t *lc = reinterpret_cast<const uint32_t *>(p.conn_local + 10L * ec);
+    const uint32_t *ps = reinterpret_cast<const uint32_t *>(p.csr_pos + 10L * ec);
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       lcw[k] = lc[k];
       psw[k] = ps[k];
     }
-    if constexpr (PHYS != PHYS_COMPAT)
-      load_state_if<R, STREAM>(p, e, q, (yw >> (t & 63)) & 1ull, pre);
   }
-  if (g >= 0) ln.stage(t, p.xm[g], p.vv[g]);
-  for (int k = t; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
+  PreState<R> pre;
+  if constexpr (PHYS != PHYS_COMPAT)
+    load_state_sel<R, STREAM>(p, ec, q, (yw >> (t & 63)) & 1ull, pre);
+  __builtin_amdgcn_sched_barrier(0);
+  if (has_n) ln.stage(t, xr, vr);
+  if (t <= p.kmax) joff[t] = jo;
   __syncthreads();
 
   EAcc ea;
@@ -416,15 +420,6 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
                                        int *wcnt, unsigned *misc) {
   extern __shared__ double smem[];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  // the node side needs only b: ids, slots and force ranges at a fixed
-  // stride, then the node records
-  int g = -1, slot = -1, deg = 0;
-  if (t < p.ml) {
-    const long o = (long)b * p.ml + t;
-    g = p.nid_pad[o];
-    slot = p.slot_pad[o];
-    deg = p.deg_pad[o];
-  }
   const RowPre row = row_prefetch(p, p.epart_base + b * (NT / 64));
   const int32_t *mt = p.meta + 8 * b;
   const int es = mt[0], nel = mt[1];
@@ -433,27 +428,38 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   uint16_t *joff = (uint16_t *)((char *)lf + lds_force_bytes(sizeof(R), 4, p.max_el));
   const double dt = p.ctl->dt;
 
-  const bool has_e = t < nel;
-  const int e = es + t;
-  uint2 lc4 = make_uint2(0u, 0u), ps4 = make_uint2(0u, 0u);
-  PreState<R> pre;
   // yielded bits of this wave's elements: a block has ywords words, one
   // per 64 elements of the largest block, and a wave past them (a block
   // plan of at most 320 elements per block) has no elements and no word
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
-  const bool has_word = p.yielded && wv < p.ywords;
+  const bool has_word = PHYS != PHYS_COMPAT && p.yielded && wv < p.ywords;
   const long yi = (long)b * p.ywords + wv;
   unsigned long long yw = ~0ull;
-  if constexpr (PHYS != PHYS_COMPAT)
-    if (has_word) yw = p.yielded[yi];
-  if (has_e) {
-    lc4 = *(const uint2 *)(p.conn_local + 4L * e);
-    ps4 = *(const uint2 *)(p.csr_pos + 4L * e);
-    if constexpr (PHYS != PHYS_COMPAT) load_state_if<R>(p, e, 0, (yw >> (t & 63)) & 1ull, pre);
-  }
-  const bool has_n = g >= 0;
-  if (has_n) ln.stage(t, p.xm[g], p.vv[g]);
-  if (t <= p.kmax) joff[t] = p.off_pad[(long)b * (p.kmax + 1) + t];
+  if (has_word) yw = p.yielded[yi];
+  // The loads of the workgroup, issued with no branch between them in the
+  // order they are needed: the node ids, slots and degrees (fixed stride,
+  // the record addresses depend on them), then the node records and the
+  // slot offsets the first barrier waits for, then the connectivity and
+  // the stress state, which stay in flight across the barrier until the
+  // element pass uses them.  Lanes past the block's nodes or elements load
+  // the last valid entry and drop it.  One offset per thread covers the
+  // block: kmax (the largest contribution count) <= max_el <= NT.
+  const long o = (long)b * p.ml + (t < p.ml ? t : p.ml - 1);
+  const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
+  const bool has_n = t < p.ml && g_o >= 0;
+  const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
+  const Node4 xr = p.xm[has_n ? g_o : 0], vr = p.vv[has_n ? g_o : 0];
+  const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
+  __builtin_amdgcn_sched_barrier(0);
+  const bool has_e = t < nel;
+  const int e = es + t, ec = has_e ? e : es + nel - 1;
+  const uint2 lc4 = *(const uint2 *)(p.conn_local + 4L * ec);
+  const uint2 ps4 = *(const uint2 *)(p.csr_pos + 4L * ec);
+  PreState<R> pre;
+  if constexpr (PHYS != PHYS_COMPAT) load_state_sel<R>(p, ec, 0, (yw >> (t & 63)) & 1ull, pre);
+  __builtin_amdgcn_sched_barrier(0);
+  if (has_n) ln.stage(t, xr, vr);
+  if (t <= p.kmax) joff[t] = jo;
   if constexpr (WALK)
     if (t == 0) misc[0] = misc[1] = 0u;
   __syncthreads();
