@@ -458,6 +458,11 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       return fail(c, CPI_ERR_INVALID, "partition: " + e);
     if (c->plan.max_local_nodes > mn)
       return fail(c, CPI_ERR_INVALID, "partition: a block holds more nodes than its kernel");
+    // the per-point tet10 kernel reads each corner as a 16-bit word (local
+    // node < 512, contribution rank < 128), which its limits of 128
+    // elements and 512 nodes per block guarantee
+    if (npe == 10 && c->nq == 4 && !c->plan.corner_fits())
+      return fail(c, CPI_ERR_INVALID, "partition: a tet10 block exceeds its corner words");
     c->eperm = c->plan.elem_perm;
     c->nperm = c->plan.node_perm;
     c->ninv = c->plan.node_inv;
@@ -560,11 +565,13 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     if (npe == 10 && c->nq == 4 && 4 * B.max_block_elems > elem_block_threads(10, c->rule))
       return fail(c, CPI_ERR_INVALID, "partition: a tet10 block has more points than threads");
     int32_t *meta, *bn, *bs, *sp, *dn, *ds;
-    uint16_t *cl, *ci, *dg, *dof;
+    uint16_t *cl, *ci, *cw = nullptr, *dg, *dof;
     Node4 *part;
+    const bool words = npe == 10 && c->nq == 4;
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||
         (rc = dalloc(c, &ci, B.csr_pos.size())) || (rc = upload(c, ci, B.csr_pos)) ||
+        (words && ((rc = dalloc(c, &cw, B.corner.size())) || (rc = upload(c, cw, B.corner)))) ||
         (rc = dalloc(c, &bn, B.bnd_nodes.size())) || (rc = upload(c, bn, B.bnd_nodes)) ||
         (rc = dalloc(c, &bs, B.bnd_slot.size())) || (rc = upload(c, bs, B.bnd_slot)) ||
         (rc = dalloc(c, &sp, B.slot_ptr.size())) || (rc = upload(c, sp, B.slot_ptr)) ||
@@ -577,6 +584,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     P.meta = meta;
     P.conn_local = cl;
     P.csr_pos = ci;
+    P.corner = cw;
     P.bnd_nodes = bn;
     P.bnd_slot = bs;
     P.slot_ptr = sp;
@@ -1023,18 +1031,21 @@ extern "C" int cpi_get_info(cpi_ctx *c, cpi_info *info) {
   info->num_partial_slots = c->scatter == CPI_SCATTER_BLOCK ? c->plan.n_slots : 0;
   // algorithmic bytes of an elastic step: connectivity 4 npe and the
   // stress read and written, 12 nq s, per element (ep is written only when
-  // a point yields; the exact-return tet4 block kernel reads it only for
-  // elements that have yielded, the other kernels read it always), then
-  // per node what each pass must touch once
+  // a point yields; the exact-return tet4 and per-point tet10 block kernels
+  // read it only for points that have yielded, the other kernels read it
+  // always), then per node what each pass must touch once
   const double s = c->prec == CPI_PREC_F32 ? 4 : 8;
-  const bool ep_read = !(c->npe == 4 && c->scatter == CPI_SCATTER_BLOCK && c->phys == CPI_PHYSICS_EP);
+  const bool block = c->scatter == CPI_SCATTER_BLOCK;
+  const bool bits = block && c->phys == CPI_PHYSICS_EP && (c->npe == 4 || c->nq == 4);
   const double state =
-      c->phys == CPI_PHYSICS_COMPAT ? 0.0 : (12.0 + (ep_read ? 1.0 : 0.0)) * c->nq * s;
+      c->phys == CPI_PHYSICS_COMPAT ? 0.0 : (12.0 + (bits ? 0.0 : 1.0)) * c->nq * s;
   const double per_elem = 4.0 * c->npe + state;
   info->bytes_per_step = c->ne * per_elem + 224.0 * c->nn;
-  if (c->scatter == CPI_SCATTER_BLOCK) {
+  if (block) {
+    // the per-point tet10 kernel reads one 16-bit word per corner
     const double nb = c->plan.n_boundary, ni = c->plan.n_interior;
-    info->elem_kernel_bytes = c->ne * per_elem + ni * 104.0 + nb * 72.0;
+    const double conn = (c->npe == 10 && c->nq == 4) ? 2.0 * c->npe : 4.0 * c->npe;
+    info->elem_kernel_bytes = c->ne * (conn + state) + ni * 104.0 + nb * 72.0;
     info->node_kernel_bytes = nb * 128.0;
   } else {
     info->elem_kernel_bytes = c->ne * per_elem + c->nn * 72.0;

@@ -82,6 +82,9 @@ struct Params {
   const int32_t *meta;
   const uint16_t *conn_local;
   const uint16_t *csr_pos;   // [E][npe]: corner -> force position in its block
+  // [E][npe]: corner as l | k << 9, its force at off[k] + l (per-point
+  // tet10 block kernel)
+  const uint16_t *corner;
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;
   const int32_t *slot_ptr;

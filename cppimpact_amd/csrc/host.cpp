@@ -354,6 +354,7 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
   P.elem_perm.assign(ne, 0);
   P.conn_local.assign((size_t)ne * npe, 0);
   P.csr_pos.assign((size_t)ne * npe, 0);
+  P.corner.assign((size_t)ne * npe, 0);
   P.bnd_nodes.clear();
   P.bnd_slot.clear();
   P.max_local_nodes = 0;
@@ -419,8 +420,10 @@ bool build_block_plan(int ne, int npe, const int32_t *conn, int nn,
       P.elem_perm[es + t] = e;
       for (int j = 0; j < npe; j++) {
         const int l = local[conn[(size_t)npe * e + j]];
+        const int k = seen[l]++;
         P.conn_local[(size_t)(es + t) * npe + j] = (uint16_t)l;
-        P.csr_pos[(size_t)(es + t) * npe + j] = (uint16_t)(off[seen[l]++] + l);
+        P.csr_pos[(size_t)(es + t) * npe + j] = (uint16_t)(off[k] + l);
+        P.corner[(size_t)(es + t) * npe + j] = (uint16_t)((l & 511) | ((k & 127) << 9));
       }
     }
     P.max_local_nodes = std::max(P.max_local_nodes, nl);

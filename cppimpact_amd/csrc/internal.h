@@ -47,6 +47,12 @@ struct BlockPlan {
   // element order, is at off[k] + l; corner j of element t writes to
   // csr_pos[t*npe + j].
   std::vector<uint16_t> csr_pos;     // [E][npe] position of each corner force
+  // The same corner in 16 bits, l | k << 9 (k: the element's rank among
+  // local node l's contributions, so csr_pos = off[k] + l), for the
+  // per-point tet10 block kernel; exact when max_local_nodes <= 512 and
+  // kmax <= 128 (corner_fits).
+  std::vector<uint16_t> corner;
+  bool corner_fits() const { return max_local_nodes <= 512 && kmax <= 128; }
   // fixed-stride per-block tables, local node l of block b at b*ml + l:
   std::vector<int32_t> nid_pad;      // device node id, -1 past the block's nodes
   std::vector<int32_t> slot_pad;     // partial slot, -1 for interior nodes

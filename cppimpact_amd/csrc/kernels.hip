@@ -338,17 +338,13 @@ k_elem_block10(Params p) {
   __builtin_amdgcn_sched_barrier(0);
   const bool has_e = (t >> 2) < nel;
   const int e = es + (t >> 2), ec = has_e ? e : es + nel - 1;
-  // the element's ten local node ids and corner slots, two per word (the
-  // rows of 20 bytes are 4-byte aligned)
-  uint32_t lcw[5], psw[5];
+  // the element's ten corner words (local node | rank << 9), two per word
+  // (the rows of 20 bytes are 4-byte aligned)
+  uint32_t cw[5];
   {
-    const uint32_t *lc = reinterpret_cast<const uint32_t *>(p.conn_local + 10L * ec);
-    const uint32_t *ps = reinterpret_cast<const uint32_t *>(p.csr_pos + 10L * ec);
+    const uint32_t *cp = reinterpret_cast<const uint32_t *>(p.corner + 10L * ec);
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
-      lcw[k] = lc[k];
-      psw[k] = ps[k];
-    }
+    for (int k = 0; k < 5; k++) cw[k] = cp[k];
   }
   PreState<R> pre;
   if constexpr (PHYS != PHYS_COMPAT)
@@ -360,13 +356,19 @@ k_elem_block10(Params p) {
 
   EAcc ea;
   if (has_e) {
-    auto half = [](const uint32_t (&w)[5], int k) {
-      return (int)((w[k >> 1] >> (16 * (k & 1))) & 0xffffu);
-    };
-    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(half(lcw, k), xk, vk); };
+    auto half = [&](int k) { return (int)((cw[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
+    auto get = [&](int k, double xk[3], double vk[3]) { ln.get(half(k) & 511, xk, vk); };
+    // lane q stores corners q, q + 4 and q + 8: their force slots
+    int sl[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      const uint32_t wj = (q >> 1) ? cw[j == 2 ? 4 : 2 * j + 1] : cw[2 * j];
+      const int h = (int)((wj >> (16 * (q & 1))) & 0xffffu);
+      sl[j] = (int)joff[h >> 9] + (h & 511);
+    }
     auto put = [&](int k, const R f[3]) {
       if ((k & 3) == q) {
-        R *dst = lf + 3 * half(psw, k);
+        R *dst = lf + 3 * sl[k >> 2];
         dst[0] = f[0];
         dst[1] = f[1];
         dst[2] = f[2];

@@ -6,6 +6,7 @@
 // UndefinedBehaviorSanitizer against cppimpact_amd/csrc/host.cpp.
 //
 // usage: plan_invariants  (runs the built-in meshes; exit status 0 = pass)
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -102,6 +103,12 @@ static void check(const std::string &label, int ne, int npe, const std::vector<i
         REQUIRE(r < km && pos == off[r] + l, "element %d corner %d slot %d != off[%d] + %d", e,
                 k, pos, r, l);
         REQUIRE(pos < nslots_lds && !pos_seen[pos]++, "LDS slot %d written twice", pos);
+        // the 16-bit corner word: local node and rank, the same slot
+        if (P.corner_fits()) {
+          const int w = P.corner[(size_t)e * npe + k];
+          REQUIRE((w & 511) == l && (w >> 9) == r && off[w >> 9] + (w & 511) == pos,
+                  "element %d corner %d word %d != %d | %d << 9", e, k, w, l, r);
+        }
       }
     }
     for (int l = 0; l < nl; l++) {
@@ -142,9 +149,42 @@ static void check(const std::string &label, int ne, int npe, const std::vector<i
   }
   for (int d = 0; d < P.n_interior; d++)
     REQUIRE(blocks_of_node[d] == 1, "interior node %d in %d blocks", d, blocks_of_node[d]);
-  std::printf("ok %s: %d elements, %d %s blocks, %d interior + %d boundary nodes, %d slots\n",
+  std::printf("ok %s: %d elements, %d %s blocks, %d interior + %d boundary nodes, %d slots, "
+              "kmax %d, corner words %s\n",
               label.c_str(), ne, nb, P.grown ? "grown" : "Hilbert", P.n_interior, P.n_boundary,
-              P.n_slots);
+              P.n_slots, P.kmax, P.corner_fits() ? "fit" : "do not fit");
+}
+
+// a fan of 2m tet4 around node 0: ring nodes 1..m on the unit circle,
+// apexes m+1 (z = 1) and m+2 (z = -1); node 0 is in every element
+static void fan(const std::string &label, int m, int max_el, bool fits) {
+  const int nn = m + 3, ne = 2 * m;
+  std::vector<double> x(3 * (size_t)nn, 0.0);
+  for (int i = 0; i < m; i++) {
+    x[3 * (1 + i)] = std::cos(2.0 * M_PI * i / m);
+    x[3 * (1 + i) + 1] = std::sin(2.0 * M_PI * i / m);
+  }
+  x[3 * (m + 1) + 2] = 1.0;
+  x[3 * (m + 2) + 2] = -1.0;
+  std::vector<int32_t> c;
+  for (int i = 0; i < m; i++) {
+    const int a = 1 + i, b = 1 + (i + 1) % m;
+    c.insert(c.end(), {0, a, b, m + 1});
+    c.insert(c.end(), {0, b, a, m + 2});
+  }
+  BlockPlan P;
+  std::string err;
+  if (!build_block_plan(ne, 4, c.data(), nn, x.data(), max_el, 384, P, err)) {
+    std::fprintf(stderr, "FAIL %s: %s\n", label.c_str(), err.c_str());
+    failures++;
+    return;
+  }
+  if (P.corner_fits() != fits) {
+    std::fprintf(stderr, "FAIL %s: kmax %d, corner words %s\n", label.c_str(), P.kmax,
+                 P.corner_fits() ? "fit" : "do not fit");
+    failures++;
+  }
+  check(label, ne, 4, c, nn, x, max_el, 384);
 }
 
 static void box(const std::string &label, int n, int quad, double jitter, int max_el,
@@ -191,6 +231,10 @@ int main() {
   box("tet10 (37, 200)", 4, 1, 0.0, 37, 200);
   box("tet4 one block", 2, 0, 0.0, 384, 384);
   cylinder("cylinder tet10 (128, 911)", 48, 6, 3, 128, 911);
+  // a node in 200 elements: one 384-element block exceeds the 16-bit
+  // corner words' 128 ranks, 128-element blocks fit them
+  fan("tet4 fan of 200 (384, 384)", 100, 384, false);
+  fan("tet4 fan of 200 (128, 384)", 100, 128, true);
   // a node of no element (left out of every block)
   {
     cpi_box_spec s{};

@@ -139,6 +139,42 @@ def test_tet10_block_size_option():
     ref["dyn"].close()
 
 
+def _fan(m, r=0.05, z0=0.05001):
+    """2m tet4 around node 0: ring nodes 1..m at radius r, apexes m+1 above
+    and m+2 below; node 0 is in every element (positive volumes)."""
+    th = 2.0 * np.pi * np.arange(m) / m
+    x = np.zeros((m + 3, 3))
+    x[1:m + 1, 0], x[1:m + 1, 1] = r * np.cos(th), r * np.sin(th)
+    x[m + 1, 2], x[m + 2, 2] = r, -r
+    x[:, 2] += z0
+    conn = []
+    for i in range(m):
+        a, b = 1 + i, 1 + (i + 1) % m
+        conn += [[0, a, b, m + 1], [0, b, a, m + 2]]
+    return x, np.asarray(conn, dtype=np.int32)
+
+
+def test_node_in_200_elements_of_a_tet4_block():
+    """A fan whose centre is in all 200 elements: one block whose centre node
+    sums 200 corner forces (the tet4 kernel's node sum has no cap on the
+    contribution count) and 64-element blocks that split them, both like the
+    oracle over a 30 m/s impact onto the wall, long enough to yield."""
+    x, conn = _fan(100)
+    k = wall_k(0.1, 4)
+    v0, dt, n = (0, 0, -30.0), 2e-8, 800
+    o = run_oracle(x, conn, v0, dt, n, O.EP, k)
+    assert o["epbar"].max() > 0, "the case must yield"
+    for be in (0, 64):
+        g = run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, block_elements=be)
+        nb = g["dyn"].info().num_blocks
+        assert nb == 1 if be == 0 else nb > 1
+        _, ep = g["dyn"].element_state()
+        assert _disp_err(g, o, x) < 1e-9
+        assert rel(g["v"], o["v"]) < 1e-9
+        assert rel(ep, o["epbar"]) < 1e-7
+        g["dyn"].close()
+
+
 @pytest.mark.parametrize("precision", [L.PREC_F64, L.PREC_F32])
 def test_tet10_streamed_state_gives_the_same_bits(precision):
     """Non-temporal stress-state accesses (chosen by size for large states,

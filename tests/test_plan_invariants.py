@@ -25,7 +25,9 @@ def test_block_plan_invariants_under_sanitizers(tmp_path):
     p = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
     assert p.returncode == 0, p.stdout + p.stderr
     assert "all plans hold" in p.stdout
-    assert p.stdout.count("ok ") == 8
+    assert p.stdout.count("ok ") == 10
+    # the 16-bit corner words fit every plan but the fan's 384-element block
+    assert p.stdout.count("corner words fit") == 9
     # both partitions are exercised: the cut Hilbert order on the boxes, the
     # grown blocks on the thin cylinder
     assert " Hilbert blocks" in p.stdout and " grown blocks" in p.stdout
