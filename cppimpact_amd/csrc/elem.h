@@ -89,10 +89,12 @@ __device__ __forceinline__ void store_state(const Params &p, int e, int q, const
 
 // State of one point loaded ahead of use (the tet4 workgroup kernel issues
 // it before its first barrier so the load overlaps the node staging);
-// point_stress leaves the updated ep in it.
+// point_stress leaves the updated ep in it, and with DEFER the updated
+// stress too, for the caller to store (changed: ep changed).
 template <typename R> struct PreState {
   R s[6];
   R ep;
+  bool changed;
 };
 
 // The stress of one point, and its ep only when the point has yielded
@@ -134,7 +136,7 @@ __device__ __forceinline__ void load_state_sel(const Params &p, int e, int q, bo
 // sigma_n + D de, then the return map (exact or surrogate).  Adds the
 // point's internal work and plastic dissipation (times wd) to ea.  (The
 // surrogate's return map is applied by the tet4 block kernel's walk.)
-template <typename R, int PHYS, bool PRE = false, bool STREAM = false>
+template <typename R, int PHYS, bool PRE = false, bool STREAM = false, bool DEFER = false>
 __device__ __forceinline__ void point_stress(const Params &p, int e, int q, const R de[6],
                                              R wd, R s[6], EAcc &ea,
                                              PreState<R> *pre = nullptr) {
@@ -179,7 +181,13 @@ __device__ __forceinline__ void point_stress(const Params &p, int e, int q, cons
     ea.wint += double(wd * R(0.5) * w);
     if (dg > R(0))
       ea.wpl += double(wd * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-    store_state<R, STREAM>(p, e, q, s, ep, dg > R(0));
+    if constexpr (PRE && DEFER) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) pre->s[c] = s[c];
+      pre->changed = dg > R(0);
+    } else {
+      store_state<R, STREAM>(p, e, q, s, ep, dg > R(0));
+    }
     if constexpr (PRE) pre->ep = ep;
   }
 }
@@ -453,7 +461,11 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 // gradient dN_i = J^-T G_i and sigma . dN_i.  Each corner force is then
 // summed over the quad's lanes as (q0 + q1) + (q2 + q3) (quad_sum, DPP
 // inside the quad), so every lane holds corner i's element force and hands
-// it to put(i, f).  STREAM: the point's state is stored non-temporal.
+// it to put(i, f).  The updated state is left in pre (ep changed in
+// pre.changed) for the caller to store after everything else it does
+// (STREAM: non-temporal): registers that a store reads must not be reused
+// before the store has completed, so a store issued here would make the
+// rest of the kernel wait for it.
 template <int RULE, typename R, int PHYS, bool STREAM, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
@@ -621,7 +633,7 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   for (int k = 0; k < 6; k++) de[k] *= R(dt);
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
-  point_stress<R, PHYS, true, STREAM>(p, e, q, de, wd, s, ea, &pre);
+  point_stress<R, PHYS, true, STREAM, true>(p, e, q, de, wd, s, ea, &pre);
   // M[r][k] = sum_t (w detJ sigma)[r][t] Ji[k][t], sigma rows (s0 s3 s5),
   // (s3 s1 s4), (s5 s4 s2)
   R M[9], S[3];

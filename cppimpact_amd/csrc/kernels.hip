@@ -355,6 +355,10 @@ k_elem_block10(Params p) {
   __syncthreads();
 
   EAcc ea;
+  // set where the point is computed: a lane without one never reads its
+  // loaded state again, so nothing after the element pass waits for the
+  // state loads (nor, by the in-order wait count, for the state stores)
+  bool yielded = false;
   if (has_e) {
     auto half = [&](int k) { return (int)((cw[k >> 1] >> (16 * (k & 1))) & 0xffffu); };
     auto get = [&](int k, double xk[3], double vk[3]) { ln.get(half(k) & 511, xk, vk); };
@@ -375,9 +379,10 @@ k_elem_block10(Params p) {
       }
     };
     tet10_point_lane<RULE, R, PHYS, STREAM>(p, get, dt, e, q, pre, put, ea);
+    yielded = pre.ep > R(0);
   }
   if constexpr (PHYS == PHYS_EP) {
-    const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
+    const unsigned long long nb = __ballot(yielded);
     if (has_word && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
   __syncthreads();
@@ -394,6 +399,9 @@ k_elem_block10(Params p) {
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
+  // the point's state last (tet10_point_lane leaves it in pre)
+  if constexpr (PHYS != PHYS_COMPAT)
+    if (has_e) store_state<R, STREAM>(p, e, q, pre.s, pre.ep, pre.changed);
 }
 
 // tet4: the plan gives every workgroup at most NT elements and NT local
@@ -479,6 +487,8 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   };
   [[maybe_unused]] R Ji[9], de[6], wd, snew[6], wk;
   [[maybe_unused]] bool yielding = false;
+  // set where the element is computed (as in k_elem_block10)
+  [[maybe_unused]] bool has_ep = false;
   if (has_e) {
     if constexpr (PHYS == PHYS_EP_MLP) {
       // an element whose trial stress stays inside the yield surface
@@ -495,10 +505,11 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
       }
     } else {
       tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+      if constexpr (PHYS == PHYS_EP) has_ep = pre.ep > R(0);
     }
   }
   if constexpr (PHYS == PHYS_EP) {
-    const unsigned long long nb = __ballot(has_e && pre.ep > R(0));
+    const unsigned long long nb = __ballot(has_ep);
     if (has_word && (t & 63) == 0 && nb != yw) p.yielded[yi] = nb;
   }
   if constexpr (PHYS == PHYS_EP_MLP) {
