@@ -486,7 +486,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
     for (int c = 0; c < 3; c++) dst[c] = fk[c];
   };
   [[maybe_unused]] R Ji[9], de[6], wd, snew[6], wk;
-  [[maybe_unused]] bool yielding = false;
+  [[maybe_unused]] bool yielding = false, ep_changed = false;
   // set where the element is computed (as in k_elem_block10)
   [[maybe_unused]] bool has_ep = false;
   if (has_e) {
@@ -543,10 +543,11 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
         }
         return;
       }
+      // stored here, not last: held through the node pass, the stress
+      // would not fit the elastic pass's 64 VGPRs
       if (has_e) store_state<R>(p, e, 0, snew, pre.ep, false);
     } else {
       int how = MLP_ELASTIC;
-      if (has_e && !yielding) store_state<R>(p, e, 0, snew, pre.ep, false);
       if (yielding) {
         // the element again from the LDS node records (the same bits as
         // above; counted there), so nothing of it stays live across the
@@ -565,8 +566,11 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
         ea.wint += double(wy * R(0.5) * wky);
         if (dg > R(0))
           ea.wpl += double(wy * R(0.81649658092772603273) * (R(p.sy0) + R(p.H) * ep) * dg);
-        store_state<R>(p, e, 0, sy, ep);
         tet4_forces<R>(Jy, sy, wy, put);
+#pragma unroll
+        for (int c = 0; c < 6; c++) snew[c] = sy[c];
+        pre.ep = ep;
+        ep_changed = true;
       }
       mlp_count(misc, how == MLP_ACCEPTED, how == MLP_FALLBACK);
       __syncthreads();
@@ -591,6 +595,13 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
+  // the walk stores the element's state last: registers that a store
+  // reads must not be reused before it completes, so a store issued
+  // earlier would hold up the node pass (the exact-return body stores it
+  // in point_stress: held to the end, the FP64 kernel ran 1-2% slower)
+  if constexpr (PHYS == PHYS_EP_MLP && WALK) {
+    if (has_e) store_state<R>(p, e, 0, snew, pre.ep, ep_changed);
+  }
 }
 
 template <typename R, int PHYS, bool AUX, int NT, int WPE>
