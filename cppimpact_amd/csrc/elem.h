@@ -461,12 +461,12 @@ __device__ __forceinline__ void tet10_points(const Params &p, const Get &get, do
 // gradient dN_i = J^-T G_i and sigma . dN_i.  Each corner force is then
 // summed over the quad's lanes as (q0 + q1) + (q2 + q3) (quad_sum, DPP
 // inside the quad), so every lane holds corner i's element force and hands
-// it to put(i, f).  The updated state is left in pre (ep changed in
-// pre.changed) for the caller to store after everything else it does
-// (STREAM: non-temporal): registers that a store reads must not be reused
-// before the store has completed, so a store issued here would make the
-// rest of the kernel wait for it.
-template <int RULE, typename R, int PHYS, bool STREAM, class Get, class Put>
+// it to put(i, f).  DEFER: the updated state is left in pre (ep changed
+// in pre.changed) for the caller to store after everything else it does,
+// else it is stored here (STREAM: non-temporal): registers that a store
+// reads must not be reused before the store has completed, so a store
+// issued here makes the rest of the kernel wait for it.
+template <int RULE, typename R, int PHYS, bool STREAM, bool DEFER, class Get, class Put>
 __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get, double dt,
                                                  int e, int q, PreState<R> &pre, const Put &put,
                                                  EAcc &ea) {
@@ -633,7 +633,7 @@ __device__ __forceinline__ void tet10_point_lane(const Params &p, const Get &get
   for (int k = 0; k < 6; k++) de[k] *= R(dt);
   const R wd = R(Q::w(q)) * detJ;
   R s[6];
-  point_stress<R, PHYS, true, STREAM, true>(p, e, q, de, wd, s, ea, &pre);
+  point_stress<R, PHYS, true, STREAM, DEFER>(p, e, q, de, wd, s, ea, &pre);
   // M[r][k] = sum_t (w detJ sigma)[r][t] Ji[k][t], sigma rows (s0 s3 s5),
   // (s3 s1 s4), (s5 s4 s2)
   R M[9], S[3];

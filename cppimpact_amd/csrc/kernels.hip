@@ -378,7 +378,7 @@ k_elem_block10(Params p) {
         dst[2] = f[2];
       }
     };
-    tet10_point_lane<RULE, R, PHYS, STREAM>(p, get, dt, e, q, pre, put, ea);
+    tet10_point_lane<RULE, R, PHYS, STREAM, !AUX>(p, get, dt, e, q, pre, put, ea);
     yielded = pre.ep > R(0);
   }
   if constexpr (PHYS == PHYS_EP) {
@@ -399,8 +399,9 @@ k_elem_block10(Params p) {
     }
   }
   wave_flush<AUX>(p, p.epart_base + b * (NT / 64), ea, row);
-  // the point's state last (tet10_point_lane leaves it in pre)
-  if constexpr (PHYS != PHYS_COMPAT)
+  // the point's state last (tet10_point_lane leaves it in pre), except in
+  // the once-per-batch variant, whose outputs leave no registers for it
+  if constexpr (PHYS != PHYS_COMPAT && !AUX)
     if (has_e) store_state<R, STREAM>(p, e, q, pre.s, pre.ep, pre.changed);
 }
 
