@@ -297,15 +297,16 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 // tet10 on a 4-point rule: four lanes per element, lane t & 3 taking
 // quadrature point t & 3, so a workgroup of NT threads holds at most NT/4
 // elements, and the plan gives it at most NT local nodes, one per thread.
-// As in the tet4 kernel every global load of the workgroup (the point's
-// stress state, its ep only when the point's yielded bit says it has
-// yielded, the element's local node ids and corner slots, the node's
-// id, slot and contribution count, the node records) is issued before the
+// As in the tet4 kernel every global load of the workgroup (the node's id,
+// slot and contribution count, the node records and slot offsets, the
+// element's ten corner words, the point's stress state with its ep only
+// when the point's yielded bit says it has yielded) is issued before the
 // first barrier; after it the element pass reads only LDS and the node pass
 // only LDS and its registers.  Corner k's element force (summed over the
 // element's four lanes in rule order) is stored by lane k & 3 into the
-// corner's jagged-diagonal slot; the nodes are then assembled as in
-// k_elem_block.  LDS as k_elem_block.
+// corner's jagged-diagonal slot, off[rank] + local node; the nodes are then
+// assembled as in k_elem_block, and the point's state is stored last.  LDS
+// as k_elem_block.
 template <int RULE, typename R, int PHYS, bool AUX, int NT, int WPE, bool STREAM>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block10(Params p) {
