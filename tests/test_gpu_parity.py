@@ -282,6 +282,29 @@ def test_reference_fuselage_mesh_compat(golden):
     g["dyn"].close()
 
 
+@pytest.mark.parametrize("name,npe", [("linear_tet.inp", 4), ("quad_tet.inp", 10)])
+def test_single_element_meshes_like_the_oracle(golden, name, npe):
+    """The reference's one-element meshes, scaled to 10 mm, thrown at the
+    wall at 30 m/s until they yield: a block of one element (lanes and
+    node slots past it load the last valid entry and drop it) in every
+    scatter, against the oracle."""
+    mesh = ci.Mesh(npe)
+    assert mesh.load_mesh(os.path.join(golden, name)) == 0
+    x, conn = np.asarray(mesh.xloc) * 1e-3, np.asarray(mesh.element_nodes)
+    wall = x[:, 2].min() - 1e-7
+    v0, dt, n = (0, 0, -30.0), 1e-8, 300
+    o = run_oracle(x, conn, v0, dt, n, O.EP, 10 * E * 0.01, wall_location=wall)
+    assert o["epbar"].max() > 0, "the case must yield"
+    for scatter in SCATTERS:
+        g = run_gpu(x, conn, v0, dt, n, 10 * E * 0.01, wall_location=wall, physics=L.PHYSICS_EP,
+                    scatter=scatter)
+        _, ep = g["dyn"].element_state()
+        assert _disp_err(g, o, x) < 1e-9, scatter
+        assert rel(g["v"], o["v"]) < 1e-9, scatter
+        assert rel(ep, o["epbar"]) < 1e-7, scatter
+        g["dyn"].close()
+
+
 def test_solve_loop_count_matches_reference_time_loop():
     # Dynamics::solve runs while (time <= time_end) with time += dt
     x, conn = cube(3)
