@@ -318,15 +318,16 @@ __device__ __forceinline__ void tet4_forces(const R Ji[9], const R s[6], R wd, c
 // tet4: B is constant, every quadrature point carries the same strain, so
 // the rule enters only through its weight sum (compat) and the element keeps
 // one stress point (ep).  Corner forces reach put() as soon as known.
+// STREAM: the state is read and written non-temporal.
 // ---------------------------------------------------------------------------
-template <typename R, int PHYS, bool PRE, class Get, class Put>
+template <typename R, int PHYS, bool PRE, bool STREAM = false, class Get, class Put>
 __device__ __forceinline__ void tet4_core(const Params &p, const Get &get, double dt, int e,
                                           const Put &put, EAcc &ea,
                                           PreState<R> *pre = nullptr) {
   R Ji[9], de[6], wd;
   tet4_kin<R, PHYS>(p, get, dt, Ji, de, wd, ea);
   R s[6];
-  point_stress<R, PHYS, PRE>(p, e, 0, de, wd, s, ea, pre);
+  point_stress<R, PHYS, PRE, STREAM>(p, e, 0, de, wd, s, ea, pre);
   tet4_forces<R>(Ji, s, wd, put);
 }
 

@@ -427,7 +427,7 @@ k_elem_block10(Params p) {
 // A deferred block computes the same trial stresses, hence the same
 // yielding set and queue order, in both launches, and which workgroup
 // finishes a block never changes its results.
-template <typename R, int PHYS, bool AUX, int NT, bool WALK>
+template <typename R, int PHYS, bool AUX, int NT, bool WALK, bool STREAM>
 __device__ __forceinline__ void block4(const Params &p, const int b, const unsigned qbase,
                                        int *wcnt, unsigned *misc) {
   extern __shared__ double smem[];
@@ -468,7 +468,8 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   const uint2 lc4 = *(const uint2 *)(p.conn_local + 4L * ec);
   const uint2 ps4 = *(const uint2 *)(p.csr_pos + 4L * ec);
   PreState<R> pre;
-  if constexpr (PHYS != PHYS_COMPAT) load_state_sel<R>(p, ec, 0, (yw >> (t & 63)) & 1ull, pre);
+  if constexpr (PHYS != PHYS_COMPAT)
+    load_state_sel<R, STREAM>(p, ec, 0, (yw >> (t & 63)) & 1ull, pre);
   __builtin_amdgcn_sched_barrier(0);
   if (has_n) ln.stage(t, xr, vr);
   if (t <= p.kmax) joff[t] = jo;
@@ -506,7 +507,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
         yielding = true;
       }
     } else {
-      tet4_core<R, PHYS, true>(p, get, dt, e, put, ea, &pre);
+      tet4_core<R, PHYS, true, STREAM>(p, get, dt, e, put, ea, &pre);
       if constexpr (PHYS == PHYS_EP) has_ep = pre.ep > R(0);
     }
   }
@@ -547,7 +548,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
       }
       // stored here, not last: held through the node pass, the stress
       // would not fit the elastic pass's 64 VGPRs
-      if (has_e) store_state<R>(p, e, 0, snew, pre.ep, false);
+      if (has_e) store_state<R, STREAM>(p, e, 0, snew, pre.ep, false);
     } else {
       int how = MLP_ELASTIC;
       if (yielding) {
@@ -602,16 +603,16 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   // earlier would hold up the node pass (the exact-return body stores it
   // in point_stress: held to the end, the FP64 kernel ran 1-2% slower)
   if constexpr (PHYS == PHYS_EP_MLP && WALK) {
-    if (has_e) store_state<R>(p, e, 0, snew, pre.ep, ep_changed);
+    if (has_e) store_state<R, STREAM>(p, e, 0, snew, pre.ep, ep_changed);
   }
 }
 
-template <typename R, int PHYS, bool AUX, int NT, int WPE>
+template <typename R, int PHYS, bool AUX, int NT, int WPE, bool STREAM>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
   __shared__ int wcnt[NT / 64];
   __shared__ unsigned misc[2];
-  block4<R, PHYS, AUX, NT, false>(p, xcd_remap(blockIdx.x, gridDim.x), 0u, wcnt, misc);
+  block4<R, PHYS, AUX, NT, false, STREAM>(p, xcd_remap(blockIdx.x, gridDim.x), 0u, wcnt, misc);
 }
 
 // The surrogate step's third launch: one workgroup per block of the grid,
@@ -621,14 +622,14 @@ k_elem_block4(Params p) {
 // the results of a block do not).  A loop over several listed blocks per
 // workgroup holds the kernel arguments in SGPRs across iterations and
 // spills 40-50 VGPRs.
-template <bool AUX, int NT, int WPE>
+template <bool AUX, int NT, int WPE, bool STREAM>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4_walk(Params p) {
   __shared__ int wcnt[NT / 64];
   __shared__ unsigned misc[2];
   if (blockIdx.x >= (unsigned)(*p.mlp_ctr >> 32)) return;
   const int2 ent = p.mlp_list[blockIdx.x];
-  block4<float, PHYS_EP_MLP, AUX, NT, true>(p, ent.x, (unsigned)ent.y, wcnt, misc);
+  block4<float, PHYS_EP_MLP, AUX, NT, true, STREAM>(p, ent.x, (unsigned)ent.y, wcnt, misc);
 }
 
 // The network on queue rows [0, n): n from the step's counter (ctr) or
@@ -1239,42 +1240,69 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
 }
 
+// the tet4 kernels' LDS opt-in, cached or streamed state
+template <typename R, int PHYS, bool STREAM>
+static hipError_t tet4_opt_in(size_t smem) {
+  constexpr int NT = block_threads<4, RULE_CENTROID>();
+  constexpr int W = tet4_waves<R, PHYS>();
+  hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W, STREAM>, smem);
+  if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, false, NT, W, STREAM>, smem);
+  if constexpr (PHYS == PHYS_EP_MLP) {
+    if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM>, smem);
+    if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM>, smem);
+  }
+  return e;
+}
+
 // the LDS opt-in (prepare_elem_block) is done once at set-up, outside any
 // stream capture
-template <int NPE, int RULE, typename R, int PHYS>
-static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s,
-                               int part) {
-  constexpr int NT = block_threads<NPE, RULE>();
-  if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
+// the tet4 launches, with the stress state cached or streamed (STREAM)
+template <typename R, int PHYS, bool STREAM>
+static hipError_t elem_block4_t(const Params &p, bool aux, int nblocks, size_t smem,
+                                hipStream_t s, int part) {
+  constexpr int NT = block_threads<4, RULE_CENTROID>();
+  constexpr int W = tet4_waves<R, PHYS>();
+  if constexpr (PHYS == PHYS_EP_MLP) {
     // the elastic pass, the network on the queue, the walk over the
     // deferred blocks (part 1, 2, 3 alone for per-launch timing)
-    constexpr int W = tet4_waves<R, PHYS>();
     if (part == 0 || part == 1) {
       if (aux)
-        timed_launch((k_elem_block4<R, PHYS, true, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+        timed_launch((k_elem_block4<R, PHYS, true, NT, W, STREAM>), dim3(nblocks), dim3(NT), smem,
+                     s, p);
       else
-        timed_launch((k_elem_block4<R, PHYS, false, NT, W>), dim3(nblocks), dim3(NT), smem, s, p);
+        timed_launch((k_elem_block4<R, PHYS, false, NT, W, STREAM>), dim3(nblocks), dim3(NT),
+                     smem, s, p);
     }
     if (part == 0 || part == 2) {
       const int tiles = (p.ne + 15) / 16;
       const int grid = std::max(1, std::min(kMlpQueueGrid, (tiles + 3) / 4));
       timed_launch((k_mlp_queue<kMlpQueueThreads>), dim3(grid), dim3(kMlpQueueThreads), 0, s,
-                         p.mlp, p.mlp_rows, (const unsigned long long *)p.mlp_ctr, 0);
+                   p.mlp, p.mlp_rows, (const unsigned long long *)p.mlp_ctr, 0);
     }
     if (part == 0 || part == 3) {
       const int grid = nblocks;
       if (aux)
-        timed_launch((k_elem_block4_walk<true, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
-                           smem, s, p);
+        timed_launch((k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM>), dim3(grid), dim3(NT),
+                     smem, s, p);
       else
-        timed_launch((k_elem_block4_walk<false, NT, kMlpWalkWaves>), dim3(grid), dim3(NT),
-                           smem, s, p);
+        timed_launch((k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM>), dim3(grid), dim3(NT),
+                     smem, s, p);
     }
     return hipGetLastError();
-  } else if constexpr (NPE == 4) {
-    constexpr int W = tet4_waves<R, PHYS>();
-    if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W>), nblocks, NT, smem, s, p);
-    CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W>), nblocks, NT, smem, s, p);
+  } else {
+    if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W, STREAM>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W, STREAM>), nblocks, NT, smem, s, p);
+  }
+}
+
+template <int NPE, int RULE, typename R, int PHYS>
+static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t smem, hipStream_t s,
+                               int part) {
+  constexpr int NT = block_threads<NPE, RULE>();
+  if constexpr (NPE == 4) {
+    if constexpr (PHYS != PHYS_COMPAT)
+      if (p.stream_state) return elem_block4_t<R, PHYS, true>(p, aux, nblocks, smem, s, part);
+    return elem_block4_t<R, PHYS, false>(p, aux, nblocks, smem, s, part);
   } else if constexpr (Quad<RULE>::n == 4) {
     constexpr int W = tet10_waves<R>();
     if constexpr (PHYS != PHYS_COMPAT) {
@@ -1351,18 +1379,11 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
       return hipErrorInvalidValue;
     } else {
       constexpr int NT = block_threads<NPE, RULE>();
-      if constexpr (NPE == 4 && PHYS == PHYS_EP_MLP) {
-        constexpr int W = tet4_waves<R, PHYS>();
-        hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
-        if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<true, NT, kMlpWalkWaves>, smem);
-        if (e != hipSuccess) return e;
-        return lds_opt_in(k_elem_block4_walk<false, NT, kMlpWalkWaves>, smem);
-      } else if constexpr (NPE == 4) {
-        constexpr int W = tet4_waves<R, PHYS>();
-        hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W>, smem);
-        if (e != hipSuccess) return e;
-        return lds_opt_in(k_elem_block4<R, PHYS, false, NT, W>, smem);
+      if constexpr (NPE == 4) {
+        hipError_t e = tet4_opt_in<R, PHYS, false>(smem);
+        if constexpr (PHYS != PHYS_COMPAT)
+          if (e == hipSuccess) e = tet4_opt_in<R, PHYS, true>(smem);
+        return e;
       } else if constexpr (Quad<RULE>::n == 4) {
         constexpr int W = tet10_waves<R>();
         hipError_t e = lds_opt_in(k_elem_block10<RULE, R, PHYS, true, NT, W, false>, smem);

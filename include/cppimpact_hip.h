@@ -149,12 +149,13 @@ typedef struct cpi_options {
                                an integer) when it is allocated, so a read of
                                memory the context never wrote shows as a
                                non-finite state (diagnostics; default 0) */
-  int32_t stream_state;     /* the per-point tet10 kernel's stress-state
-                               accesses: 0 = non-temporal when a step's
-                               working set (state, node records, partial
-                               forces, corner words) exceeds 192 MB
-                               (default), 1 = always, 2 = never; results
-                               are the same bits */
+  int32_t stream_state;     /* the block kernels' stress-state accesses
+                               (tet4 and the 4-point tet10 kernel):
+                               0 = non-temporal when a step's working set
+                               (state, node records, partial forces,
+                               connectivity) exceeds 192 MB (default),
+                               1 = always, 2 = never; results are the
+                               same bits */
   double gravity[3];        /* default (0, 0, -9.81): dynamics_kernels.h:165-167 */
   double mlp_tolerance;     /* default 1e-3 */
 } cpi_options;

@@ -208,16 +208,18 @@ def test_surrogate_step_against_exact_return():
 
 def test_surrogate_step_is_bitwise_reproducible():
     """The surrogate kernel's queue, tile packing and counters are fixed by
-    the block, not by timing: two graph runs and an eager run of a plastic
-    impact agree bit for bit, counters included."""
+    the block, not by timing: two graph runs, an eager run and a run with
+    the stress state streamed past the caches (elastic pass and walk) of a
+    plastic impact agree bit for bit, counters included."""
     x, conn = cube(6, gap=1e-6, jitter=0.1, seed=2)
     smp = collect_samples(ci.Mesh.from_arrays(x, conn), material(),
                           ci.Wall("Wall", 0.0, wall_k(0.25, 6), None, 0, 1), [30.0], 0.0, 60)
     net = Surrogate(material(), seed=3)
     net.train(smp, epochs=4, batch=512)
     runs = []
-    for graph in (1, 1, 0):
-        d = _impact(x, conn, 30.0, stress_update=L.RETURN_MLP, use_graph=graph)
+    for graph, stream in ((1, 2), (1, 2), (0, 2), (1, 1)):
+        d = _impact(x, conn, 30.0, stress_update=L.RETURN_MLP, use_graph=graph,
+                    stream_state=stream)
         d.set_surrogate(net)
         d.init(0.0)
         d.step(60)

@@ -176,16 +176,20 @@ def test_node_in_200_elements_of_a_tet4_block():
 
 
 @pytest.mark.parametrize("precision", [L.PREC_F64, L.PREC_F32])
-def test_tet10_streamed_state_gives_the_same_bits(precision):
-    """Non-temporal stress-state accesses (chosen by size for large states,
+@pytest.mark.parametrize("npe", [10, 4])
+def test_streamed_state_gives_the_same_bits(npe, precision):
+    """Non-temporal stress-state accesses (chosen by the working set's size,
     forced here on a small mesh) change where the state is cached, not its
     values: the same bits as cached accesses over a plastic impact, and the
-    oracle's state in FP64."""
-    x, conn = cube(5, quadratic=True, jitter=0.05, seed=4, gap=1e-6)
+    oracle's state in FP64; tet10 on the 4-point kernel, tet4 on its block
+    kernel."""
+    quad = npe == 10
+    x, conn = cube(5 if quad else 8, quadratic=quad, jitter=0.05, seed=4, gap=1e-6)
     k = wall_k(0.25, 10)
     v0, dt, n = (0, 0, -40.0), 1e-7, 50
-    runs = [run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, quadrature=L.QUAD_EXACT4,
-                    precision=precision, stream_state=s) for s in (1, 2)]
+    opts = dict(quadrature=L.QUAD_EXACT4) if quad else {}
+    runs = [run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, precision=precision,
+                    stream_state=s, **opts) for s in (1, 2)]
     for key in ("x", "v", "a", "energy"):
         assert np.array_equal(runs[0][key], runs[1][key]), key
     s1, e1 = runs[0]["dyn"].element_state()
@@ -193,7 +197,7 @@ def test_tet10_streamed_state_gives_the_same_bits(precision):
     assert e1.max() > 0.0
     assert np.array_equal(s1, s2) and np.array_equal(e1, e2)
     if precision == L.PREC_F64:
-        o = run_oracle(x, conn, v0, dt, n, O.EP, k, quad_rule=L.QUAD_EXACT4)
+        o = run_oracle(x, conn, v0, dt, n, O.EP, k, **({"quad_rule": L.QUAD_EXACT4} if quad else {}))
         assert _disp_err(runs[0], o, x) < 1e-9 and rel(runs[0]["v"], o["v"]) < 1e-9
     for g in runs:
         g["dyn"].close()
