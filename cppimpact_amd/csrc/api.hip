@@ -555,13 +555,13 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     // connectivity) exceeds 192 MB, three quarters of the 256 MB Infinity
     // Cache, the block kernels stream the state past the caches, which then
     // keep the node records and partials that other blocks and the node
-    // pass read again: the c3 cylinder (307 MB in FP32, 469 MB in FP64) and
-    // the 128^3 tet4 cube (0.6-1 GB) step 4-6% faster; the c2 cube (80 and
-    // 130 MB) stays cached, which streaming would make 6-9% slower
+    // pass read again: the c3 cylinder (300 MB in FP32, 455 MB in FP64) and
+    // the 128^3 tet4 cube (0.6-1 GB) step 4-6% faster; the c2 cube (91 and
+    // 140 MB) stays cached, which streaming would make 6-9% slower
     const size_t slots = c->scatter == CPI_SCATTER_BLOCK ? (size_t)c->plan.n_slots : 0;
     const size_t conn = (npe == 10 && c->nq == 4) ? 2 * (size_t)npe : 4 * (size_t)npe;
     const size_t ws = rs * 7 * c->nq * (size_t)ne + 64 * (size_t)c->nn +
-                      slots * (c->prec == CPI_PREC_F32 ? 16 : 32) + conn * ne;
+                      slots * (c->prec == CPI_PREC_F32 ? 12 : 24) + conn * ne;
     P.stream_state = opt->stream_state == 1   ? 1
                      : opt->stream_state == 2 ? 0
                                               : (ws > ((size_t)192 << 20));
@@ -574,7 +574,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       return fail(c, CPI_ERR_INVALID, "partition: a tet10 block has more points than threads");
     int32_t *meta, *bn, *bs, *sp, *dn, *ds;
     uint16_t *cl, *ci, *cw = nullptr, *dg, *dof;
-    Node4 *part;
+    Part3 *part;
     const bool words = npe == 10 && c->nq == 4;
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||

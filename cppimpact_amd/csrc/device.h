@@ -41,6 +41,16 @@ struct alignas(32) Node4 {
   double x, y, z, w;
 };
 
+// One block's partial force on a shared node: packed (x, y, z) records,
+// 24 B in FP64 and 12 B in FP32 (a padded fourth word would add a third to
+// the bytes the element kernels write and the boundary-node kernel reads)
+struct alignas(8) Part3 {
+  double x, y, z;
+};
+struct alignas(4) Part3f {
+  float x, y, z;
+};
+
 struct Params {
   int nn, ne, npe, nq;
   // material (D per tetrahedral.h:486-512: c11 = (1-nu) k, c12 = nu k,
@@ -88,7 +98,7 @@ struct Params {
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;
   const int32_t *slot_ptr;
-  Node4 *partial;        // one force partial per slot (x, y, z, -)
+  Part3 *partial;        // one force partial per slot (Part3f in FP32)
   // per-block node tables at a fixed stride ml (local node j of block b at
   // b*ml + j), read without the block's meta: device node id (-1 past the
   // block's nodes), partial slot (-1 for interior nodes), contribution

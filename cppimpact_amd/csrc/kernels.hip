@@ -178,16 +178,15 @@ __device__ __forceinline__ void node_force_sum(const R *lf, const uint16_t *joff
 }
 
 // One block's partial force on a shared node, stored in the element
-// kernel's precision: FP64 records (32 B) for FP64 elements, float4 (16 B)
-// for FP32 ones, whose element forces carry FP32 rounding already.  The
-// boundary-node kernel sums them in FP64.
+// kernel's precision: FP64 records (24 B) for FP64 elements, FP32 ones
+// (12 B) for FP32 elements, whose element forces carry FP32 rounding
+// already.  The boundary-node kernel sums them in FP64.
 template <typename R>
 __device__ __forceinline__ void store_partial(const Params &p, int slot, const double f[3]) {
   if constexpr (std::is_same<R, float>::value)
-    reinterpret_cast<float4 *>(p.partial)[slot] = make_float4(float(f[0]), float(f[1]),
-                                                              float(f[2]), 0.f);
+    reinterpret_cast<Part3f *>(p.partial)[slot] = Part3f{float(f[0]), float(f[1]), float(f[2])};
   else
-    p.partial[slot] = Node4{f[0], f[1], f[2], 0.0};
+    p.partial[slot] = Part3{f[0], f[1], f[2]};
 }
 
 // A workgroup's LDS node records, structure of arrays of ml doubles each:
@@ -661,7 +660,7 @@ __global__ void __launch_bounds__(NT) k_mlp_queue(const MlpParams *P, uint16_t *
 
 template <bool AUX, typename R>
 __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
-  using PR = typename std::conditional<std::is_same<R, float>::value, float4, Node4>::type;
+  using PR = typename std::conditional<std::is_same<R, float>::value, Part3f, Part3>::type;
   const PR *part = reinterpret_cast<const PR *>(p.partial);
   const int i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * 4);
