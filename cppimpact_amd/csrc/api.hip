@@ -116,21 +116,27 @@ double rule_weight_sum(int rule) {
   }
 }
 
-// user AoS [N][3] -> device 32-byte records (device node order), w = 0
-int put_nodes(cpi_ctx *c, Node4 *d, const double *aos) {
-  std::vector<Node4> tmp(c->nn);
+// user AoS [N][3] <-> device nodal records in device node order (Node4
+// with w = 0, or Vec3)
+template <typename T>
+int put_nodes(cpi_ctx *c, T *d, const double *aos) {
+  std::vector<T> tmp(c->nn);
   for (int i = 0; i < c->nn; i++) {
     const double *s = aos + 3L * c->nperm[i];
-    tmp[i] = Node4{s[0], s[1], s[2], 0.0};
+    tmp[i] = T{};
+    tmp[i].x = s[0];
+    tmp[i].y = s[1];
+    tmp[i].z = s[2];
   }
-  HIPCHK(c, hipMemcpyAsync(d, tmp.data(), sizeof(Node4) * c->nn, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d, tmp.data(), sizeof(T) * c->nn, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return CPI_OK;
 }
 
-int get_nodes(cpi_ctx *c, const Node4 *d, double *aos) {
-  std::vector<Node4> tmp(c->nn);
-  HIPCHK(c, hipMemcpyAsync(tmp.data(), d, sizeof(Node4) * c->nn, hipMemcpyDeviceToHost, c->stream));
+template <typename T>
+int get_nodes(cpi_ctx *c, const T *d, double *aos) {
+  std::vector<T> tmp(c->nn);
+  HIPCHK(c, hipMemcpyAsync(tmp.data(), d, sizeof(T) * c->nn, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < c->nn; i++) {
     double *o = aos + 3L * c->nperm[i];
@@ -555,12 +561,12 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     // connectivity) exceeds 192 MB, three quarters of the 256 MB Infinity
     // Cache, the block kernels stream the state past the caches, which then
     // keep the node records and partials that other blocks and the node
-    // pass read again: the c3 cylinder (300 MB in FP32, 455 MB in FP64) and
-    // the 128^3 tet4 cube (0.6-1 GB) step 4-6% faster; the c2 cube (91 and
-    // 140 MB) stays cached, which streaming would make 6-9% slower
+    // pass read again: the c3 cylinder (285 MB in FP32, 440 MB in FP64) and
+    // the 128^3 tet4 cube (0.6-1 GB) step 4-6% faster; the c2 cube (89 and
+    // 138 MB) stays cached, which streaming would make 6-9% slower
     const size_t slots = c->scatter == CPI_SCATTER_BLOCK ? (size_t)c->plan.n_slots : 0;
     const size_t conn = (npe == 10 && c->nq == 4) ? 2 * (size_t)npe : 4 * (size_t)npe;
-    const size_t ws = rs * 7 * c->nq * (size_t)ne + 64 * (size_t)c->nn +
+    const size_t ws = rs * 7 * c->nq * (size_t)ne + 56 * (size_t)c->nn +
                       slots * (c->prec == CPI_PREC_F32 ? 12 : 24) + conn * ne;
     P.stream_state = opt->stream_state == 1   ? 1
                      : opt->stream_state == 2 ? 0
@@ -574,7 +580,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
       return fail(c, CPI_ERR_INVALID, "partition: a tet10 block has more points than threads");
     int32_t *meta, *bn, *bs, *sp, *dn, *ds;
     uint16_t *cl, *ci, *cw = nullptr, *dg, *dof;
-    Part3 *part;
+    Vec3 *part;
     const bool words = npe == 10 && c->nq == 4;
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||

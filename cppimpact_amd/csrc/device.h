@@ -41,13 +41,13 @@ struct alignas(32) Node4 {
   double x, y, z, w;
 };
 
-// One block's partial force on a shared node: packed (x, y, z) records,
-// 24 B in FP64 and 12 B in FP32 (a padded fourth word would add a third to
-// the bytes the element kernels write and the boundary-node kernel reads)
-struct alignas(8) Part3 {
+// Packed (x, y, z) records, 24 B in FP64 and 12 B in FP32: velocities and
+// the blocks' partial forces on shared nodes (a padded fourth word would add
+// a third to the bytes each step reads and writes of them)
+struct alignas(8) Vec3 {
   double x, y, z;
 };
-struct alignas(4) Part3f {
+struct alignas(4) Vec3f {
   float x, y, z;
 };
 
@@ -69,7 +69,7 @@ struct Params {
   // nodal state, device node order: one 32-byte record per node per field,
   // so a gathered node costs one cache line per field
   Node4 *xm;             // (x, y, z, m)
-  Node4 *vv;             // (vx, vy, vz, 0) = v_{n+1/2}
+  Vec3 *vv;              // v_{n+1/2}
   double *m;             // lumped mass as assembled (copied into xm.w)
   double *f[3];          // assembled force (colour / atomic scatter)
   Node4 *acc;            // a_n                   written when store_aux
@@ -98,7 +98,7 @@ struct Params {
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;
   const int32_t *slot_ptr;
-  Part3 *partial;        // one force partial per slot (Part3f in FP32)
+  Vec3 *partial;        // one force partial per slot (Vec3f in FP32)
   // per-block node tables at a fixed stride ml (local node j of block b at
   // b*ml + j), read without the block's meta: device node id (-1 past the
   // block's nodes), partial slot (-1 for interior nodes), contribution

@@ -17,7 +17,7 @@
 // node); with the atomic scatter it is one launch using FP64 atomics; both
 // are followed by k_node_all.
 //
-// Nodal state lives in 32-byte records: xm = (x, y, z, m), vv = (v, 0).
+// Nodal state: 32-byte records xm = (x, y, z, m) and 24-byte ones vv = v.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -147,7 +147,7 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
     xn[c] = x[c] + du[c];
   }
   p.xm[g] = Node4{xn[0], xn[1], xn[2], m};
-  p.vv[g] = Node4{vn[0], vn[1], vn[2], 0.0};
+  p.vv[g] = Vec3{vn[0], vn[1], vn[2]};
   if (AUX) {
     p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
     p.du[g] = Node4{du[0], du[1], du[2], 0.0};
@@ -184,9 +184,9 @@ __device__ __forceinline__ void node_force_sum(const R *lf, const uint16_t *joff
 template <typename R>
 __device__ __forceinline__ void store_partial(const Params &p, int slot, const double f[3]) {
   if constexpr (std::is_same<R, float>::value)
-    reinterpret_cast<Part3f *>(p.partial)[slot] = Part3f{float(f[0]), float(f[1]), float(f[2])};
+    reinterpret_cast<Vec3f *>(p.partial)[slot] = Vec3f{float(f[0]), float(f[1]), float(f[2])};
   else
-    p.partial[slot] = Part3{f[0], f[1], f[2]};
+    p.partial[slot] = Vec3{f[0], f[1], f[2]};
 }
 
 // A workgroup's LDS node records, structure of arrays of ml doubles each:
@@ -196,7 +196,7 @@ __device__ __forceinline__ void store_partial(const Params &p, int slot, const d
 struct LNodes {
   double *d;
   int ml;
-  __device__ __forceinline__ void stage(int j, const Node4 &xm, const Node4 &vv) const {
+  __device__ __forceinline__ void stage(int j, const Node4 &xm, const Vec3 &vv) const {
     d[j] = xm.x;
     d[ml + j] = xm.y;
     d[2 * ml + j] = xm.z;
@@ -333,7 +333,8 @@ k_elem_block10(Params p) {
   const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
   const bool has_n = t < p.ml && g_o >= 0;
   const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
-  const Node4 xr = p.xm[has_n ? g_o : 0], vr = p.vv[has_n ? g_o : 0];
+  const Node4 xr = p.xm[has_n ? g_o : 0];
+  const Vec3 vr = p.vv[has_n ? g_o : 0];
   const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
   __builtin_amdgcn_sched_barrier(0);
   const bool has_e = (t >> 2) < nel;
@@ -459,7 +460,8 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
   const bool has_n = t < p.ml && g_o >= 0;
   const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
-  const Node4 xr = p.xm[has_n ? g_o : 0], vr = p.vv[has_n ? g_o : 0];
+  const Node4 xr = p.xm[has_n ? g_o : 0];
+  const Vec3 vr = p.vv[has_n ? g_o : 0];
   const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
   __builtin_amdgcn_sched_barrier(0);
   const bool has_e = t < nel;
@@ -660,7 +662,7 @@ __global__ void __launch_bounds__(NT) k_mlp_queue(const MlpParams *P, uint16_t *
 
 template <bool AUX, typename R>
 __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
-  using PR = typename std::conditional<std::is_same<R, float>::value, Part3f, Part3>::type;
+  using PR = typename std::conditional<std::is_same<R, float>::value, Vec3f, Vec3>::type;
   const PR *part = reinterpret_cast<const PR *>(p.partial);
   const int i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
   const RowPre row = row_prefetch(p, p.epart_base + blockIdx.x * 4);
@@ -668,7 +670,8 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
   EAcc ea;
   if (i < p.n_boundary) {
     const int g = p.n_interior + i;
-    const Node4 xm = p.xm[g], vv = p.vv[g];
+    const Node4 xm = p.xm[g];
+    const Vec3 vv = p.vv[g];
     // partials in slot order, loaded four at a time
     double f[3] = {0.0, 0.0, 0.0};
     const int s1 = p.slot_ptr[i + 1];
@@ -719,7 +722,8 @@ __global__ void __launch_bounds__(NT) k_elem_direct(Params p) {
 #pragma unroll
     for (int k = 0; k < NPE; k++) nd[k] = p.conn[(long)k * p.ne + e];
     auto get = [&](int k, double xk[3], double vk[3]) {
-      const Node4 a = p.xm[nd[k]], b = p.vv[nd[k]];
+      const Node4 a = p.xm[nd[k]];
+      const Vec3 b = p.vv[nd[k]];
       xk[0] = a.x;
       xk[1] = a.y;
       xk[2] = a.z;
@@ -754,7 +758,8 @@ __global__ void __launch_bounds__(256) k_node_all(Params p) {
       f[c] = p.f[c][g];
       p.f[c][g] = 0.0;
     }
-    const Node4 xm = p.xm[g], vv = p.vv[g];
+    const Node4 xm = p.xm[g];
+    const Vec3 vv = p.vv[g];
     const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
     node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
   }
@@ -770,7 +775,8 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
-    const Node4 xm = p.xm[g], vv = p.vv[g];
+    const Node4 xm = p.xm[g];
+    const Vec3 vv = p.vv[g];
     const double m = xm.w;
     const bool held = p.fixed_mask && p.fixed_mask[g];
     const double x[3] = {xm.x, xm.y, xm.z};
@@ -794,7 +800,7 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
       ke += v[c] * v[c];
       vn[c] = v[c] + 0.5 * dt * a[c];
     }
-    p.vv[g] = Node4{vn[0], vn[1], vn[2], 0.0};
+    p.vv[g] = Vec3{vn[0], vn[1], vn[2]};
     p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
     p.du[g] = Node4{0.0, 0.0, 0.0, 0.0};
     p.xprev[g] = xm;
