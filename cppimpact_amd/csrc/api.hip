@@ -362,6 +362,14 @@ extern "C" void cpi_destroy(cpi_ctx *c) {
   delete c;
 }
 
+// The block kernels that read each corner as a 16-bit word, l | k << 9
+// (local node, its contribution rank): the per-point tet10 kernel always,
+// the tet4 kernels when the plan's blocks fit the words.
+static bool corner_words(const cpi_ctx *c) {
+  return c->scatter == CPI_SCATTER_BLOCK &&
+         ((c->npe == 10 && c->nq == 4) || (c->npe == 4 && c->plan.corner_fits()));
+}
+
 static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *mat,
                        const cpi_wall *wall, const cpi_options *opt) {
   c->opt = *opt;
@@ -565,12 +573,14 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     // the 128^3 tet4 cube (0.6-1 GB) step 4-6% faster; the c2 cube (89 and
     // 138 MB) stays cached, which streaming would make 6-9% slower
     const size_t slots = c->scatter == CPI_SCATTER_BLOCK ? (size_t)c->plan.n_slots : 0;
-    const size_t conn = (npe == 10 && c->nq == 4) ? 2 * (size_t)npe : 4 * (size_t)npe;
+    const size_t conn = corner_words(c) ? 2 * (size_t)npe : 4 * (size_t)npe;
     const size_t ws = rs * 7 * c->nq * (size_t)ne + 56 * (size_t)c->nn +
                       slots * (c->prec == CPI_PREC_F32 ? 12 : 24) + conn * ne;
     P.stream_state = opt->stream_state == 1   ? 1
                      : opt->stream_state == 2 ? 0
                                               : (ws > ((size_t)192 << 20));
+    // the tet4 kernels of plans beyond the corner words keep it cached
+    if (npe == 4 && c->scatter == CPI_SCATTER_BLOCK && !corner_words(c)) P.stream_state = 0;
   }
   if (c->scatter == CPI_SCATTER_BLOCK) {
     BlockPlan &B = c->plan;
@@ -581,7 +591,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
     int32_t *meta, *bn, *bs, *sp, *dn, *ds;
     uint16_t *cl, *ci, *cw = nullptr, *dg, *dof;
     Vec3 *part;
-    const bool words = npe == 10 && c->nq == 4;
+    const bool words = corner_words(c);
     if ((rc = dalloc(c, &meta, B.meta.size())) || (rc = upload(c, meta, B.meta)) ||
         (rc = dalloc(c, &cl, B.conn_local.size())) || (rc = upload(c, cl, B.conn_local)) ||
         (rc = dalloc(c, &ci, B.csr_pos.size())) || (rc = upload(c, ci, B.csr_pos)) ||
@@ -1056,9 +1066,9 @@ extern "C" int cpi_get_info(cpi_ctx *c, cpi_info *info) {
   const double per_elem = 4.0 * c->npe + state;
   info->bytes_per_step = c->ne * per_elem + 224.0 * c->nn;
   if (block) {
-    // the per-point tet10 kernel reads one 16-bit word per corner
+    // corners as 16-bit words, or two u16 arrays
     const double nb = c->plan.n_boundary, ni = c->plan.n_interior;
-    const double conn = (c->npe == 10 && c->nq == 4) ? 2.0 * c->npe : 4.0 * c->npe;
+    const double conn = corner_words(c) ? 2.0 * c->npe : 4.0 * c->npe;
     info->elem_kernel_bytes = c->ne * (conn + state) + ni * 104.0 + nb * 72.0;
     info->node_kernel_bytes = nb * 128.0;
   } else {

@@ -427,7 +427,12 @@ k_elem_block10(Params p) {
 // A deferred block computes the same trial stresses, hence the same
 // yielding set and queue order, in both launches, and which workgroup
 // finishes a block never changes its results.
-template <typename R, int PHYS, bool AUX, int NT, bool WALK, bool STREAM>
+// CW: the element's corners as 16-bit words l | k << 9 (p.corner, 8 B per
+// element), the slot off[k] + l taken from the LDS offsets after the
+// barrier; otherwise the local ids and slots as two u16 arrays (16 B), for
+// plans whose blocks exceed the words (a node in more than 128 elements of
+// one block).
+template <typename R, int PHYS, bool AUX, int NT, bool WALK, bool STREAM, bool CW>
 __device__ __forceinline__ void block4(const Params &p, const int b, const unsigned qbase,
                                        int *wcnt, unsigned *misc) {
   extern __shared__ double smem[];
@@ -466,8 +471,9 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   __builtin_amdgcn_sched_barrier(0);
   const bool has_e = t < nel;
   const int e = es + t, ec = has_e ? e : es + nel - 1;
-  const uint2 lc4 = *(const uint2 *)(p.conn_local + 4L * ec);
-  const uint2 ps4 = *(const uint2 *)(p.csr_pos + 4L * ec);
+  const uint2 lc4 = *(const uint2 *)((CW ? p.corner : p.conn_local) + 4L * ec);
+  uint2 ps4 = make_uint2(0u, 0u);
+  if constexpr (!CW) ps4 = *(const uint2 *)(p.csr_pos + 4L * ec);
   PreState<R> pre;
   if constexpr (PHYS != PHYS_COMPAT)
     load_state_sel<R, STREAM>(p, ec, 0, (yw >> (t & 63)) & 1ull, pre);
@@ -479,13 +485,38 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   __syncthreads();
 
   EAcc ea;
-  int l[4] = {(int)(lc4.x & 0xffffu), (int)(lc4.x >> 16), (int)(lc4.y & 0xffffu),
-              (int)(lc4.y >> 16)};
-  int ps[4] = {(int)(ps4.x & 0xffffu), (int)(ps4.x >> 16), (int)(ps4.y & 0xffffu),
-               (int)(ps4.y >> 16)};
-  auto get = [&](int k, double xk[3], double vk[3]) { ln.get(l[k], xk, vk); };
+  // corner k's 16-bit field of the two loaded words: its local id for the
+  // gather and its slot for the force, decoded where they are used, or up
+  // front in the walk (each form the one with fewer VGPRs)
+  auto half = [&](const uint2 &u, int k) -> int {
+    const unsigned v = k < 2 ? u.x : u.y;
+    return (int)((k & 1) ? (v >> 16) : (v & 0xffffu));
+  };
+  auto lid = [&](int k) -> int { return CW ? (half(lc4, k) & 511) : half(lc4, k); };
+  auto pos = [&](int k) -> int {
+    const int h = half(lc4, k);
+    return CW ? joff[h >> 9] + (h & 511) : half(ps4, k);
+  };
+  [[maybe_unused]] int l[4], ps[4];
+  if constexpr (WALK) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      l[k] = lid(k);
+      ps[k] = pos(k);
+    }
+  }
+  auto get = [&](int k, double xk[3], double vk[3]) {
+    if constexpr (WALK)
+      ln.get(l[k], xk, vk);
+    else
+      ln.get(lid(k), xk, vk);
+  };
   auto put = [&](int k, const R fk[3]) {
-    R *dst = lf + 3 * ps[k];
+    R *dst;
+    if constexpr (WALK)
+      dst = lf + 3 * ps[k];
+    else
+      dst = lf + 3 * pos(k);
 #pragma unroll
     for (int c = 0; c < 3; c++) dst[c] = fk[c];
   };
@@ -608,12 +639,12 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   }
 }
 
-template <typename R, int PHYS, bool AUX, int NT, int WPE, bool STREAM>
+template <typename R, int PHYS, bool AUX, int NT, int WPE, bool STREAM, bool CW>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4(Params p) {
   __shared__ int wcnt[NT / 64];
   __shared__ unsigned misc[2];
-  block4<R, PHYS, AUX, NT, false, STREAM>(p, xcd_remap(blockIdx.x, gridDim.x), 0u, wcnt, misc);
+  block4<R, PHYS, AUX, NT, false, STREAM, CW>(p, xcd_remap(blockIdx.x, gridDim.x), 0u, wcnt, misc);
 }
 
 // The surrogate step's third launch: one workgroup per block of the grid,
@@ -623,14 +654,14 @@ k_elem_block4(Params p) {
 // the results of a block do not).  A loop over several listed blocks per
 // workgroup holds the kernel arguments in SGPRs across iterations and
 // spills 40-50 VGPRs.
-template <bool AUX, int NT, int WPE, bool STREAM>
+template <bool AUX, int NT, int WPE, bool STREAM, bool CW>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_elem_block4_walk(Params p) {
   __shared__ int wcnt[NT / 64];
   __shared__ unsigned misc[2];
   if (blockIdx.x >= (unsigned)(*p.mlp_ctr >> 32)) return;
   const int2 ent = p.mlp_list[blockIdx.x];
-  block4<float, PHYS_EP_MLP, AUX, NT, true, STREAM>(p, ent.x, (unsigned)ent.y, wcnt, misc);
+  block4<float, PHYS_EP_MLP, AUX, NT, true, STREAM, CW>(p, ent.x, (unsigned)ent.y, wcnt, misc);
 }
 
 // The network on queue rows [0, n): n from the step's counter (ctr) or
@@ -1245,24 +1276,28 @@ static hipError_t lds_opt_in(K kernel, size_t smem) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
 }
 
-// the tet4 kernels' LDS opt-in, cached or streamed state
-template <typename R, int PHYS, bool STREAM>
+// the tet4 kernels' LDS opt-in, cached or streamed state, corner words or
+// the two u16 arrays
+template <typename R, int PHYS, bool STREAM, bool CW>
 static hipError_t tet4_opt_in(size_t smem) {
   constexpr int NT = block_threads<4, RULE_CENTROID>();
   constexpr int W = tet4_waves<R, PHYS>();
-  hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W, STREAM>, smem);
-  if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, false, NT, W, STREAM>, smem);
+  hipError_t e = lds_opt_in(k_elem_block4<R, PHYS, true, NT, W, STREAM, CW>, smem);
+  if (e == hipSuccess) e = lds_opt_in(k_elem_block4<R, PHYS, false, NT, W, STREAM, CW>, smem);
   if constexpr (PHYS == PHYS_EP_MLP) {
-    if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM>, smem);
-    if (e == hipSuccess) e = lds_opt_in(k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM>, smem);
+    if (e == hipSuccess)
+      e = lds_opt_in(k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM, CW>, smem);
+    if (e == hipSuccess)
+      e = lds_opt_in(k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM, CW>, smem);
   }
   return e;
 }
 
 // the LDS opt-in (prepare_elem_block) is done once at set-up, outside any
 // stream capture
-// the tet4 launches, with the stress state cached or streamed (STREAM)
-template <typename R, int PHYS, bool STREAM>
+// the tet4 launches, with the stress state cached or streamed (STREAM),
+// the corners as 16-bit words or two u16 arrays (CW)
+template <typename R, int PHYS, bool STREAM, bool CW>
 static hipError_t elem_block4_t(const Params &p, bool aux, int nblocks, size_t smem,
                                 hipStream_t s, int part) {
   constexpr int NT = block_threads<4, RULE_CENTROID>();
@@ -1272,10 +1307,10 @@ static hipError_t elem_block4_t(const Params &p, bool aux, int nblocks, size_t s
     // deferred blocks (part 1, 2, 3 alone for per-launch timing)
     if (part == 0 || part == 1) {
       if (aux)
-        timed_launch((k_elem_block4<R, PHYS, true, NT, W, STREAM>), dim3(nblocks), dim3(NT), smem,
+        timed_launch((k_elem_block4<R, PHYS, true, NT, W, STREAM, CW>), dim3(nblocks), dim3(NT), smem,
                      s, p);
       else
-        timed_launch((k_elem_block4<R, PHYS, false, NT, W, STREAM>), dim3(nblocks), dim3(NT),
+        timed_launch((k_elem_block4<R, PHYS, false, NT, W, STREAM, CW>), dim3(nblocks), dim3(NT),
                      smem, s, p);
     }
     if (part == 0 || part == 2) {
@@ -1287,16 +1322,16 @@ static hipError_t elem_block4_t(const Params &p, bool aux, int nblocks, size_t s
     if (part == 0 || part == 3) {
       const int grid = nblocks;
       if (aux)
-        timed_launch((k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM>), dim3(grid), dim3(NT),
+        timed_launch((k_elem_block4_walk<true, NT, kMlpWalkWaves, STREAM, CW>), dim3(grid), dim3(NT),
                      smem, s, p);
       else
-        timed_launch((k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM>), dim3(grid), dim3(NT),
+        timed_launch((k_elem_block4_walk<false, NT, kMlpWalkWaves, STREAM, CW>), dim3(grid), dim3(NT),
                      smem, s, p);
     }
     return hipGetLastError();
   } else {
-    if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W, STREAM>), nblocks, NT, smem, s, p);
-    CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W, STREAM>), nblocks, NT, smem, s, p);
+    if (aux) CPI_LAUNCH((k_elem_block4<R, PHYS, true, NT, W, STREAM, CW>), nblocks, NT, smem, s, p);
+    CPI_LAUNCH((k_elem_block4<R, PHYS, false, NT, W, STREAM, CW>), nblocks, NT, smem, s, p);
   }
 }
 
@@ -1305,9 +1340,12 @@ static hipError_t elem_block_t(const Params &p, bool aux, int nblocks, size_t sm
                                int part) {
   constexpr int NT = block_threads<NPE, RULE>();
   if constexpr (NPE == 4) {
+    // a plan whose blocks exceed the corner words has none uploaded, and
+    // its state cached
+    if (!p.corner) return elem_block4_t<R, PHYS, false, false>(p, aux, nblocks, smem, s, part);
     if constexpr (PHYS != PHYS_COMPAT)
-      if (p.stream_state) return elem_block4_t<R, PHYS, true>(p, aux, nblocks, smem, s, part);
-    return elem_block4_t<R, PHYS, false>(p, aux, nblocks, smem, s, part);
+      if (p.stream_state) return elem_block4_t<R, PHYS, true, true>(p, aux, nblocks, smem, s, part);
+    return elem_block4_t<R, PHYS, false, true>(p, aux, nblocks, smem, s, part);
   } else if constexpr (Quad<RULE>::n == 4) {
     constexpr int W = tet10_waves<R>();
     if constexpr (PHYS != PHYS_COMPAT) {
@@ -1385,9 +1423,10 @@ template <int NPE, int RULE, typename R, int PHYS> struct ElemBlockPrepF {
     } else {
       constexpr int NT = block_threads<NPE, RULE>();
       if constexpr (NPE == 4) {
-        hipError_t e = tet4_opt_in<R, PHYS, false>(smem);
+        hipError_t e = tet4_opt_in<R, PHYS, false, true>(smem);
+        if (e == hipSuccess) e = tet4_opt_in<R, PHYS, false, false>(smem);
         if constexpr (PHYS != PHYS_COMPAT)
-          if (e == hipSuccess) e = tet4_opt_in<R, PHYS, true>(smem);
+          if (e == hipSuccess) e = tet4_opt_in<R, PHYS, true, true>(smem);
         return e;
       } else if constexpr (Quad<RULE>::n == 4) {
         constexpr int W = tet10_waves<R>();
