@@ -155,7 +155,9 @@ typedef struct cpi_options {
                                (state, node records, partial forces,
                                connectivity) exceeds 192 MB (default),
                                1 = always, 2 = never; results are the
-                               same bits */
+                               same bits.  tet4 plans with a node in more
+                               than 128 elements of one block keep the
+                               state cached */
   double gravity[3];        /* default (0, 0, -9.81): dynamics_kernels.h:165-167 */
   double mlp_tolerance;     /* default 1e-3 */
 } cpi_options;
