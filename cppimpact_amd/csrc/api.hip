@@ -246,28 +246,22 @@ int compute_mass(cpi_ctx *c) {
       e = launch_mass(p, c->rule, 1, c->mass_kind, rho, 0, c->stream);
     }
   }
-  if (e == hipSuccess) e = launch_pack_mass(p, c->stream);
   if (e != hipSuccess) return fail(c, CPI_ERR_HIP, std::string("mass kernel: ") + hipGetErrorString(e));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->mass_ready = true;
   return CPI_OK;
 }
 
-// x = X0 (mass kept in xm.w when assembled), v = v0, forces/aux/state zeroed
+// x = X0, v = v0, forces/aux/state zeroed (the mass, once assembled, stays)
 int reset_state(cpi_ctx *c) {
-  int rc = put_nodes(c, c->P.xm, c->X0u.data());
+  int rc = put_nodes(c, c->P.pos, c->X0u.data());
   if (!rc) rc = put_nodes(c, c->P.vv, c->V0u.data());
+  if (!rc) rc = put_nodes(c, c->P.xprev, c->X0u.data());
   if (rc) return rc;
-  if (c->mass_ready) {
-    hipError_t e = launch_pack_mass(c->P, c->stream);
-    if (e != hipSuccess) return fail(c, CPI_ERR_HIP, hipGetErrorString(e));
-  }
   for (int k = 0; k < 3; k++)
     HIPCHK(c, hipMemsetAsync(c->P.f[k], 0, sizeof(double) * c->nn, c->stream));
   HIPCHK(c, hipMemsetAsync(c->P.acc, 0, sizeof(Node4) * c->nn, c->stream));
   HIPCHK(c, hipMemsetAsync(c->P.du, 0, sizeof(Node4) * c->nn, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->P.xprev, c->P.xm, sizeof(Node4) * c->nn, hipMemcpyDeviceToDevice,
-                           c->stream));
   const size_t rs = c->prec == CPI_PREC_F32 ? 4 : 8;
   if (c->phys != CPI_PHYSICS_COMPAT) {
     HIPCHK(c, hipMemsetAsync(c->P.sig, 0, rs * 6 * c->nq * (size_t)c->ne, c->stream));
@@ -519,7 +513,7 @@ static int create_impl(cpi_ctx *c, const cpi_mesh_desc *m, const cpi_material *m
   for (int kk = 0; kk < 3 && !rc; kk++) {
     rc = dalloc(c, &P.f[kk], nn);
   }
-  if (!rc) rc = dalloc(c, &P.xm, nn);
+  if (!rc) rc = dalloc(c, &P.pos, nn);
   if (!rc) rc = dalloc(c, &P.vv, nn);
   if (!rc) rc = dalloc(c, &P.acc, nn);
   if (!rc) rc = dalloc(c, &P.du, nn);
@@ -922,7 +916,7 @@ extern "C" int cpi_get_state(cpi_ctx *c, double *x, double *v, double *a, double
   if (!c) return CPI_ERR_INVALID;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int rc = CPI_OK;
-  if (x && (rc = get_nodes(c, c->P.xm, x))) return rc;
+  if (x && (rc = get_nodes(c, c->P.pos, x))) return rc;
   std::vector<double> hv, ha;
   if (v || vi) {
     hv.resize(3L * c->nn);

@@ -66,11 +66,11 @@ struct Params {
   double qw[kMaxQ];
   double qpt[kMaxQ][3];
   double wsum;
-  // nodal state, device node order: one 32-byte record per node per field,
-  // so a gathered node costs one cache line per field
-  Node4 *xm;             // (x, y, z, m)
+  // nodal state, device node order: one record per node per field, so a
+  // gathered node's x (or v) costs one or two 32-byte sectors
+  Vec3 *pos;             // x
   Vec3 *vv;              // v_{n+1/2}
-  double *m;             // lumped mass as assembled (copied into xm.w)
+  double *m;             // lumped mass (read by the node's owner only)
   double *f[3];          // assembled force (colour / atomic scatter)
   Node4 *acc;            // a_n                   written when store_aux
   Node4 *du;             // du of the step        written when store_aux

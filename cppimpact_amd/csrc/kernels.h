@@ -33,7 +33,6 @@ int energy_reduce_blocks();  // rows of p.ered (EP_N doubles each)
 // lumped mass into p.m (scatter: 0 block, 1 colour range, 2 atomic)
 hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double rho,
                        int nblocks, hipStream_t s);
-hipError_t launch_pack_mass(const Params &p, hipStream_t s);
 hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s);
 hipError_t launch_stress_elem(const Params &p, double *out, hipStream_t s);
 hipError_t launch_debug_strain(const Params &p, int rule, double alpha, int dcase, double nu,

@@ -17,7 +17,8 @@
 // node); with the atomic scatter it is one launch using FP64 atomics; both
 // are followed by k_node_all.
 //
-// Nodal state: 32-byte records xm = (x, y, z, m) and 24-byte ones vv = v.
+// Nodal state: 24-byte records pos = x and vv = v, and the lumped mass m
+// (read only by the nodes' owners: a step never writes it).
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -146,7 +147,7 @@ __device__ __forceinline__ void node_update(const Params &p, int g, const double
     wg += p.grav[c] * du[c];
     xn[c] = x[c] + du[c];
   }
-  p.xm[g] = Node4{xn[0], xn[1], xn[2], m};
+  p.pos[g] = Vec3{xn[0], xn[1], xn[2]};
   p.vv[g] = Vec3{vn[0], vn[1], vn[2]};
   if (AUX) {
     p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
@@ -196,11 +197,11 @@ __device__ __forceinline__ void store_partial(const Params &p, int slot, const d
 struct LNodes {
   double *d;
   int ml;
-  __device__ __forceinline__ void stage(int j, const Node4 &xm, const Vec3 &vv) const {
-    d[j] = xm.x;
-    d[ml + j] = xm.y;
-    d[2 * ml + j] = xm.z;
-    d[3 * ml + j] = xm.w;
+  __device__ __forceinline__ void stage(int j, const Vec3 &x, double m, const Vec3 &vv) const {
+    d[j] = x.x;
+    d[ml + j] = x.y;
+    d[2 * ml + j] = x.z;
+    d[3 * ml + j] = m;
     d[4 * ml + j] = vv.x;
     d[5 * ml + j] = vv.y;
     d[6 * ml + j] = vv.z;
@@ -245,7 +246,7 @@ __global__ void __launch_bounds__(NT) k_elem_block(Params p) {
 
   for (int j = threadIdx.x; j < ml; j += NT) {
     const int g = nid[j];
-    if (g >= 0) ln.stage(j, p.xm[g], p.vv[g]);
+    if (g >= 0) ln.stage(j, p.pos[g], p.m[g], p.vv[g]);
   }
   for (int k = threadIdx.x; k <= p.kmax; k += NT) joff[k] = p.off_pad[(long)b * (p.kmax + 1) + k];
   __syncthreads();
@@ -333,7 +334,8 @@ k_elem_block10(Params p) {
   const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
   const bool has_n = t < p.ml && g_o >= 0;
   const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
-  const Node4 xr = p.xm[has_n ? g_o : 0];
+  const Vec3 xr = p.pos[has_n ? g_o : 0];
+  const double mr = p.m[has_n && slot_o < 0 ? g_o : 0];
   const Vec3 vr = p.vv[has_n ? g_o : 0];
   const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
   __builtin_amdgcn_sched_barrier(0);
@@ -351,7 +353,7 @@ k_elem_block10(Params p) {
   if constexpr (PHYS != PHYS_COMPAT)
     load_state_sel<R, STREAM>(p, ec, q, (yw >> (t & 63)) & 1ull, pre);
   __builtin_amdgcn_sched_barrier(0);
-  if (has_n) ln.stage(t, xr, vr);
+  if (has_n) ln.stage(t, xr, mr, vr);
   if (t <= p.kmax) joff[t] = jo;
   __syncthreads();
 
@@ -465,7 +467,8 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
   const bool has_n = t < p.ml && g_o >= 0;
   const int g = has_n ? g_o : -1, slot = has_n ? slot_o : -1, deg = has_n ? deg_o : 0;
-  const Node4 xr = p.xm[has_n ? g_o : 0];
+  const Vec3 xr = p.pos[has_n ? g_o : 0];
+  const double mr = p.m[has_n && slot_o < 0 ? g_o : 0];
   const Vec3 vr = p.vv[has_n ? g_o : 0];
   const uint16_t jo = p.off_pad[(long)b * (p.kmax + 1) + (t < p.kmax ? t : p.kmax)];
   __builtin_amdgcn_sched_barrier(0);
@@ -478,7 +481,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   if constexpr (PHYS != PHYS_COMPAT)
     load_state_sel<R, STREAM>(p, ec, 0, (yw >> (t & 63)) & 1ull, pre);
   __builtin_amdgcn_sched_barrier(0);
-  if (has_n) ln.stage(t, xr, vr);
+  if (has_n) ln.stage(t, xr, mr, vr);
   if (t <= p.kmax) joff[t] = jo;
   if constexpr (WALK)
     if (t == 0) misc[0] = misc[1] = 0u;
@@ -701,8 +704,8 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
   EAcc ea;
   if (i < p.n_boundary) {
     const int g = p.n_interior + i;
-    const Node4 xm = p.xm[g];
-    const Vec3 vv = p.vv[g];
+    const Vec3 xr = p.pos[g], vv = p.vv[g];
+    const double m = p.m[g];
     // partials in slot order, loaded four at a time
     double f[3] = {0.0, 0.0, 0.0};
     const int s1 = p.slot_ptr[i + 1];
@@ -726,8 +729,8 @@ __global__ void __launch_bounds__(256) k_node_bnd(Params p) {
       s += 2;
     }
     if (s < s1) add(part[s]);
-    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
-    node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
+    const double x[3] = {xr.x, xr.y, xr.z}, v[3] = {vv.x, vv.y, vv.z};
+    node_update<AUX>(p, g, f, x, v, m, dt, ea);
   }
   wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea, row);
   // the last launch of the step: the surrogate's queue and block list start
@@ -753,8 +756,7 @@ __global__ void __launch_bounds__(NT) k_elem_direct(Params p) {
 #pragma unroll
     for (int k = 0; k < NPE; k++) nd[k] = p.conn[(long)k * p.ne + e];
     auto get = [&](int k, double xk[3], double vk[3]) {
-      const Node4 a = p.xm[nd[k]];
-      const Vec3 b = p.vv[nd[k]];
+      const Vec3 a = p.pos[nd[k]], b = p.vv[nd[k]];
       xk[0] = a.x;
       xk[1] = a.y;
       xk[2] = a.z;
@@ -789,10 +791,9 @@ __global__ void __launch_bounds__(256) k_node_all(Params p) {
       f[c] = p.f[c][g];
       p.f[c][g] = 0.0;
     }
-    const Node4 xm = p.xm[g];
-    const Vec3 vv = p.vv[g];
-    const double x[3] = {xm.x, xm.y, xm.z}, v[3] = {vv.x, vv.y, vv.z};
-    node_update<AUX>(p, g, f, x, v, xm.w, dt, ea);
+    const Vec3 xr = p.pos[g], vv = p.vv[g];
+    const double x[3] = {xr.x, xr.y, xr.z}, v[3] = {vv.x, vv.y, vv.z};
+    node_update<AUX>(p, g, f, x, v, p.m[g], dt, ea);
   }
   wave_flush<AUX>(p, p.epart_base + blockIdx.x * 4, ea, row);
   if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->step += 1;
@@ -806,11 +807,10 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
   const double dt = p.ctl->dt;
   EAcc ea;
   if (g < p.nn) {
-    const Node4 xm = p.xm[g];
-    const Vec3 vv = p.vv[g];
-    const double m = xm.w;
+    const Vec3 xr = p.pos[g], vv = p.vv[g];
+    const double m = p.m[g];
     const bool held = p.fixed_mask && p.fixed_mask[g];
-    const double x[3] = {xm.x, xm.y, xm.z};
+    const double x[3] = {xr.x, xr.y, xr.z};
     const double v[3] = {held ? 0.0 : vv.x, held ? 0.0 : vv.y, held ? 0.0 : vv.z};
     double a[3] = {0.0, 0.0, 0.0}, ke = 0.0, vn[3];
     if (p.wall_on && !held && (!p.contact_mask || p.contact_mask[g])) {
@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(256) k_node_init(Params p) {
     p.vv[g] = Vec3{vn[0], vn[1], vn[2]};
     p.acc[g] = Node4{a[0], a[1], a[2], 0.0};
     p.du[g] = Node4{0.0, 0.0, 0.0, 0.0};
-    p.xprev[g] = xm;
+    p.xprev[g] = Node4{xr.x, xr.y, xr.z, m};
     ea.ke = 0.5 * m * ke;
   }
   wave_flush<true>(p, blockIdx.x * 4, ea, row);
@@ -916,7 +916,7 @@ __global__ void __launch_bounds__(64) k_energy_reduce2(Params p) {
 }
 
 // ---------------------------------------------------------------------------
-// lumped mass (into the scratch array m, then packed into xm.w)
+// lumped mass, into m
 // ---------------------------------------------------------------------------
 template <int NPE, int RULE, int NT>
 __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rho) {
@@ -932,7 +932,7 @@ __global__ void __launch_bounds__(NT) k_mass_block(Params p, int kind, double rh
   for (int j = threadIdx.x; j < ml; j += NT) {
     const int g = nid[j];
     if (g >= 0) {
-      const Node4 q = p.xm[g];
+      const Vec3 q = p.pos[g];
       lx[j] = q.x;
       lx[ml + j] = q.y;
       lx[2 * ml + j] = q.z;
@@ -986,7 +986,7 @@ __global__ void __launch_bounds__(256) k_mass_direct(Params p, int kind, double 
 #pragma unroll
   for (int k = 0; k < NPE; k++) {
     nd[k] = p.conn[(long)k * p.ne + e];
-    const Node4 q = p.xm[nd[k]];
+    const Vec3 q = p.pos[nd[k]];
     xs[k][0] = q.x;
     xs[k][1] = q.y;
     xs[k][2] = q.z;
@@ -1001,11 +1001,6 @@ __global__ void __launch_bounds__(256) k_mass_direct(Params p, int kind, double 
   }
 }
 
-__global__ void __launch_bounds__(256) k_pack_mass(Params p) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g < p.nn) p.xm[g].w = p.m[g];
-}
-
 // ---------------------------------------------------------------------------
 // stable dt of the current configuration, one minimum per workgroup
 // ---------------------------------------------------------------------------
@@ -1018,7 +1013,7 @@ __global__ void __launch_bounds__(256) k_elem_dt(Params p, double *out) {
     double xs[4][3];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const Node4 q = p.xm[p.conn[(long)k * p.ne + e]];
+      const Vec3 q = p.pos[p.conn[(long)k * p.ne + e]];
       xs[k][0] = q.x;
       xs[k][1] = q.y;
       xs[k][2] = q.z;
@@ -1531,10 +1526,6 @@ hipError_t launch_mass(const Params &p, int rule, int scatter, int kind, double 
     case RULE_EXACT4: return mass_rule_t<10, RULE_EXACT4>(p, scatter, kind, rho, nblocks, s);
     default: return mass_rule_t<10, RULE_REF5>(p, scatter, kind, rho, nblocks, s);
   }
-}
-
-hipError_t launch_pack_mass(const Params &p, hipStream_t s) {
-  CPI_LAUNCH(k_pack_mass, (p.nn + 255) / 256, 256, 0, s, p);
 }
 
 hipError_t launch_elem_dt(const Params &p, double *out, hipStream_t s) {
