@@ -85,15 +85,16 @@ struct Params {
   // t%64); ep of the others is 0 and is not read.  Null: ep is always read.
   unsigned long long *yielded;
   int ywords;
-  // 1: the per-point tet10 kernel reads and writes the stress state with
-  // non-temporal accesses (a state larger than the last-level cache)
+  // 1: the tet4 and per-point tet10 block kernels read and write the stress
+  // state with non-temporal accesses (a working set larger than the cache)
   int stream_state;
   // workgroup plan (LDS-assembled scatter)
   const int32_t *meta;
   const uint16_t *conn_local;
   const uint16_t *csr_pos;   // [E][npe]: corner -> force position in its block
-  // [E][npe]: corner as l | k << 9, its force at off[k] + l (per-point
-  // tet10 block kernel)
+  // [E][npe]: corner as l | k << 9, its force at off[k] + l (the per-point
+  // tet10 block kernel; the tet4 ones when the plan fits the words, else
+  // null and they read conn_local and csr_pos)
   const uint16_t *corner;
   const int32_t *bnd_nodes;
   const int32_t *bnd_slot;

@@ -328,7 +328,7 @@ k_elem_block10(Params p) {
   unsigned long long yw = ~0ull;
   if (has_word) yw = p.yielded[yi];
   // the loads in the order of the tet4 body (block4): node ids, then the
-  // node records and slot offsets, then connectivity and stress state, with
+  // node records and slot offsets, then corner words and stress state, with
   // no branch between them (one offset per thread: kmax <= max_el <= NT/4)
   const long o = (long)b * p.ml + (t < p.ml ? t : p.ml - 1);
   const int slot_o = p.slot_pad[o], deg_o = p.deg_pad[o], g_o = p.nid_pad[o];
@@ -410,7 +410,7 @@ k_elem_block10(Params p) {
 
 // tet4: the plan gives every workgroup at most NT elements and NT local
 // nodes, so each thread owns one element and at most one node, and every
-// global load of the workgroup (stress state, local connectivity, the
+// global load of the workgroup (stress state, corner words, the
 // contribution lists, node records, slots) is issued before the first
 // barrier.  LDS: node records (LNodes), corner forces [max_el*4][3] R in
 // jagged-diagonal slots, the slot offsets off[k].
@@ -458,7 +458,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
   // The loads of the workgroup, issued with no branch between them in the
   // order they are needed: the node ids, slots and degrees (fixed stride,
   // the record addresses depend on them), then the node records and the
-  // slot offsets the first barrier waits for, then the connectivity and
+  // slot offsets the first barrier waits for, then the corner words and
   // the stress state, which stay in flight across the barrier until the
   // element pass uses them.  Lanes past the block's nodes or elements load
   // the last valid entry and drop it.  One offset per thread covers the
