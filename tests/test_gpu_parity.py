@@ -157,8 +157,11 @@ def _fan(m, r=0.05, z0=0.05001):
 def test_node_in_200_elements_of_a_tet4_block():
     """A fan whose centre is in all 200 elements: one block whose centre node
     sums 200 corner forces (the tet4 kernel's node sum has no cap on the
-    contribution count) and 64-element blocks that split them, both like the
-    oracle over a 30 m/s impact onto the wall, long enough to yield."""
+    contribution count, and the block is beyond the 16-bit corner words, so
+    the kernel reads the two u16 arrays) and 64-element blocks that split
+    them (corner words), both like the oracle over a 30 m/s impact onto the
+    wall, long enough to yield.  Streaming asked for gives the same bits: the
+    one-block plan keeps its state cached, the 64-element one streams it."""
     x, conn = _fan(100)
     k = wall_k(0.1, 4)
     v0, dt, n = (0, 0, -30.0), 2e-8, 800
@@ -172,6 +175,11 @@ def test_node_in_200_elements_of_a_tet4_block():
         assert _disp_err(g, o, x) < 1e-9
         assert rel(g["v"], o["v"]) < 1e-9
         assert rel(ep, o["epbar"]) < 1e-7
+        gs = run_gpu(x, conn, v0, dt, n, k, physics=L.PHYSICS_EP, block_elements=be,
+                     stream_state=1)
+        for key in ("x", "v", "a", "energy"):
+            assert np.array_equal(gs[key], g[key]), (be, key)
+        gs["dyn"].close()
         g["dyn"].close()
 
 
