@@ -49,8 +49,9 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 // wave_sum of a term that is zero on every lane of most waves (plastic
-// work, inverted elements, penalty energy, contact force): a uniform branch
-// skips the reduction then
+// work, inverted elements, penalty energy, contact force, and the nodal
+// terms in the waves of an element kernel that update no node): a uniform
+// branch skips the reduction then
 __device__ __forceinline__ double wave_sum_sparse(double v) {
   return __ballot(v != 0.0) ? wave_sum(v) : 0.0;
 }
@@ -82,10 +83,10 @@ template <bool ALL>
 __device__ __forceinline__ void wave_flush(const Params &p, int base, const EAcc &ea,
                                            const RowPre &pre) {
   const double wint = wave_sum(ea.wint), wpl = wave_sum_sparse(ea.wpl),
-               bad = wave_sum_sparse(ea.bad), wg = wave_sum(ea.wgrav);
+               bad = wave_sum_sparse(ea.bad), wg = wave_sum_sparse(ea.wgrav);
   double ke = 0.0, pen = 0.0, fc = 0.0, md = 0.0;
   if (ALL) {
-    ke = wave_sum(ea.ke);
+    ke = wave_sum_sparse(ea.ke);
     pen = wave_sum_sparse(ea.pen);
     fc = wave_sum_sparse(ea.fc);
     md = wave_min(ea.mindt);
