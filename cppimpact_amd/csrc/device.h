@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <limits>
+#include <type_traits>
 
 namespace cpi {
 
@@ -268,6 +270,37 @@ __device__ __forceinline__ R yield_overshoot(const Params &p, const R s[6], R ep
   return nrm - k23 * (R(p.sy0) + R(p.H) * ep);
 }
 
+// A trial stress whose squared deviator norm q2 lies below the squared
+// yield radius by 64 epsilon: far more than the rounding of the square
+// root, the squares and the radius (a few epsilon), so the test
+// sqrt(q2) - radius > 0 leaves every such stress elastic as well, and
+// skipping the square root for it changes no decision and no bit.  FP64
+// only: an FP64 square root is a dozen instructions at the FP64 rate, an
+// FP32 one about one, which the test would cost as much as it saves.
+template <typename R>
+__device__ __forceinline__ bool surely_elastic(const Params &p, R q2, R ep) {
+  if constexpr (!std::is_same<R, double>::value) {
+    return false;
+  } else {
+    const R k23 = R(0.81649658092772603273);  // sqrt(2/3)
+    const R rad = k23 * (R(p.sy0) + R(p.H) * ep);
+    return q2 <= rad * rad * (R(1) - R(64) * std::numeric_limits<R>::epsilon());
+  }
+}
+
+// yield_overshoot(p, s, ep) > 0, the square root taken only near or
+// beyond the yield surface
+template <typename R>
+__device__ __forceinline__ bool trial_yields(const Params &p, const R s[6], R ep) {
+  const R third = R(1) / R(3);
+  R pm = (s[0] + s[1] + s[2]) * third;
+  R d0 = s[0] - pm, d1 = s[1] - pm, d2 = s[2] - pm;
+  const R q2 = d0 * d0 + d1 * d1 + d2 * d2 + R(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  if (surely_elastic<R>(p, q2, ep)) return false;
+  const R k23 = R(0.81649658092772603273);  // sqrt(2/3)
+  return sqrt(q2) - k23 * (R(p.sy0) + R(p.H) * ep) > R(0);
+}
+
 // J2 return map, linear isotropic hardening (Simo & Hughes 1998 Box 3.2
 // restricted to isotropic hardening); s holds the trial stress on entry.
 template <typename R>
@@ -275,8 +308,9 @@ __device__ __forceinline__ R radial_return(const Params &p, R s[6], R &ep) {
   const R third = R(1) / R(3);
   R pm = (s[0] + s[1] + s[2]) * third;
   R d0 = s[0] - pm, d1 = s[1] - pm, d2 = s[2] - pm;
-  R nrm = sqrt(d0 * d0 + d1 * d1 + d2 * d2 +
-               R(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]));
+  const R q2 = d0 * d0 + d1 * d1 + d2 * d2 + R(2) * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  if (surely_elastic<R>(p, q2, ep)) return R(0);
+  R nrm = sqrt(q2);
   const R k23 = R(0.81649658092772603273);  // sqrt(2/3)
   R fy = nrm - k23 * (R(p.sy0) + R(p.H) * ep);
   if (!(fy > R(0))) return R(0);

@@ -534,7 +534,7 @@ __device__ __forceinline__ void block4(const Params &p, const int b, const unsig
       // finishes here; a yielding one waits for the network
       tet4_kin<R, PHYS_EP_MLP>(p, get, dt, Ji, de, wd, ea);
       wk = ep_trial<R>(p, de, pre.s, snew);
-      if (!(yield_overshoot<R>(p, snew, pre.ep) > R(0))) {
+      if (!trial_yields<R>(p, snew, pre.ep)) {
 #pragma unroll
         for (int c = 0; c < 6; c++) wk += snew[c] * de[c];
         ea.wint += double(wd * R(0.5) * wk);
